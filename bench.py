@@ -1,0 +1,171 @@
+"""Benchmark: 2-D lid-driven cavity, 2049 x 2049 FP64, 50 Jacobi sweeps per step.
+
+One step = one call of the cavity kernel (b, 1 + 50 pressure sweeps with four
+boundary classes each, u, v, walls, lid) on the MI355X backend, fields resident
+in HBM.  Prints one JSON line:
+
+* value: cell-updates/s = 2049^2 x steps / wall time of the timed steps;
+* roofline: the dominant kernel (the Jacobi pressure sweep) - algorithmic bytes
+  per launch (class-0 cells x 8 B x 3 arrays: p read, b read, p written) over
+  its average launch duration, measured with HIP events on the backend stream
+  in an instrumented pass of the same steps, against 8 TB/s;
+* cpu_baseline: the oracle's restatement of the reference CPU path (gcc
+  -O3 -fopenmp) on a bounded sample of the same workload on the host cores.
+
+``--dt 1e-5`` is the configured time step; it exceeds the explicit scheme's
+stability limit, so fields overflow after a few hundred steps.  Timing does not
+depend on the values; parity tests use dt=2e-6.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=2049)
+    ap.add_argument("--dt", type=float, default=1e-5)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target length of the CPU sample")
+    ap.add_argument("--instrumented-steps", type=int, default=20)
+    ap.add_argument("--no-graphs", action="store_true")
+    ap.add_argument("--no-fuse", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this backend runs on one GPU; multi-GPU runs are not part of this build"}))
+        sys.exit(2)
+
+    import numpy as np
+    import xgrid_amd
+    from xgrid_amd import _lib
+    from xgrid_amd import workloads as W
+
+    xgrid_amd.init(precision="double", cacheroot=os.path.join(ROOT, ".xgrid"),
+                   graphs=not args.no_graphs, fuse=not args.no_fuse)
+    n = args.n
+    k = W.make_cavity()
+    cfg = W.cavity_config(n, dt=args.dt)
+    masks = W.cavity_masks(n)
+    grids = [xgrid_amd.Grid((n, n), float) for _ in "bpuv"]
+    for g, c in zip(grids, "bpuv"):
+        g.boundary[...] = masks[c]
+    lib = _lib.load()
+
+    def step():
+        k(*grids, cfg)
+
+    for _ in range(max(1, args.warmup)):
+        step()
+    native = k.native
+    _lib.check(lib.xg_synchronize())
+    native.statement_counts(reset=True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    _lib.check(lib.xg_synchronize())
+    t1 = time.perf_counter()
+    wall = t1 - t0
+    counts = native.statement_counts(reset=True)
+    ms_per_step = wall / args.steps * 1e3
+    cells = n * n
+    value = cells * args.steps / wall
+    step_bytes = native.algorithmic_bytes(counts, grids) / args.steps
+
+    # dominant kernel: the Jacobi sweep (the implicit statement inside the loop)
+    jac = [m.stmt.sid for m in native.lowered.stmts if m.stmt.mode == "pingpong"][-1]
+    jm = native.lowered.stmts[jac]
+    p_grid = grids[native.grid_index[jm.stmt.grid]]
+    jac_cells = p_grid.class_info(jm.stmt.klass)["count"]
+    jac_bytes = jac_cells * 8 * (len([r for r in jm.refs if r[1] >= 0]) + 1)
+    native.set_options(graphs=False)
+    native.time_statement(jac, True)
+    for _ in range(args.instrumented_steps):
+        step()
+    total_ms, launches = native.statement_time(jac)
+    native.time_statement(jac, False)
+    native.set_options()
+    avg_ms = total_ms / max(1, launches)
+    achieved = jac_bytes / (avg_ms * 1e-3) / 1e9
+
+    name = ctypes_name(lib)
+    out = {
+        "metric": "cell-updates/s",
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"lid-driven cavity {n}x{n} FP64, 1+50 Jacobi sweeps/step, dt={args.dt}, "
+                               "plain @xgrid.kernel calls",
+                   "grid": [n, n], "sweeps_per_step": 51, "device": name},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": f"xg_k{jac}_pp (Jacobi sweep)", "bytes_per_launch": jac_bytes,
+                     "avg_launch_ms": avg_ms, "launches_timed": launches},
+        "step_roofline": {"algorithmic_bytes_per_step": step_bytes,
+                          "achieved_GBs": step_bytes / (ms_per_step * 1e-3) / 1e9,
+                          "frac": step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS},
+        "counters": native.counters(),
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(n, args.dt, args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+def ctypes_name(lib) -> str:
+    import ctypes
+    buf = ctypes.create_string_buffer(256)
+    cus = ctypes.c_int()
+    hbm = ctypes.c_int64()
+    lib.xg_device_name(buf, 256, ctypes.byref(cus), ctypes.byref(hbm))
+    return f"{buf.value.decode()} {cus.value} CUs {hbm.value / 2**30:.0f} GiB"
+
+
+def cpu_baseline(n: int, dt: float, seconds: float) -> dict:
+    """The oracle's restated reference CPU path on the same cavity, a bounded sample."""
+    import xgrid_amd
+    from xgrid_amd import workloads as W
+    from oracle.cpu_path import CpuKernel, HostGrid
+    threads = int(os.environ.get("OMP_NUM_THREADS", len(os.sched_getaffinity(0))))
+    k = W.make_cavity()
+    ck = CpuKernel(k, opt_level=3)
+    ck.threads(threads)
+    cfg = W.cavity_config(n, dt=dt)
+    masks = W.cavity_masks(n)
+    grids = [HostGrid((n, n), float) for _ in "bpuv"]
+    for g, c in zip(grids, "bpuv"):
+        g.boundary[...] = masks[c]
+    ck(*grids, cfg)        # first call: JIT compile and first-touch of the levels
+    steps = 0
+    t0 = time.perf_counter()
+    while True:
+        ck(*grids, cfg)
+        steps += 1
+        el = time.perf_counter() - t0
+        if el >= seconds or steps >= 50:
+            break
+    return {"value": n * n * steps / el, "unit": "cell-updates/s", "cores": threads, "kind": "port",
+            "sample": f"{steps} steps of the {n}x{n} cavity ({el:.1f} s), gcc -O3 -fopenmp, "
+                      f"OMP_NUM_THREADS={threads}"}
+
+
+if __name__ == "__main__":
+    main()

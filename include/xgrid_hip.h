@@ -1,0 +1,151 @@
+/* xgrid_hip.h - C ABI of libxgrid_hip.so, the MI355X (gfx950) execution backend
+ * for xgrid stencil kernels.
+ *
+ * Every entry point returns XG_OK (0) or a negative XG_ERR_* code; the message
+ * of the last failure on the calling thread is xg_last_error().  Nothing
+ * aborts and no C++ exception crosses this boundary.  Plain pointers and sizes
+ * only.  Citations name the reference (acasta-yhliu/xgrid) interface each entry
+ * point replaces.
+ */
+#ifndef XGRID_HIP_H
+#define XGRID_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define XG_OK 0
+#define XG_ERR_HIP (-1)     /* a HIP runtime call failed */
+#define XG_ERR_RTC (-2)     /* hipRTC compilation failed (log in xg_last_error) */
+#define XG_ERR_ARG (-3)     /* invalid argument / handle */
+#define XG_ERR_MEM (-4)     /* allocation refused (device memory or budget) */
+#define XG_ERR_STATE (-5)   /* call out of order (e.g. no device initialised) */
+#define XG_ERR_IO (-6)      /* code-object cache file error */
+
+/* statement execution modes (decided from the kernel's source) */
+#define XG_MODE_INPLACE 0   /* reference K1/K3 nest, generator.py:289-304,353-361 */
+#define XG_MODE_PINGPONG 1  /* reference K2 implicit nest as a buffer swap, generator.py:312-352 */
+#define XG_MODE_TWOPHASE 2  /* reference K2 with a shifted store: compute, then store */
+
+#define XG_DESC_MAX_REFS 16
+#define XG_DESC_MAX_HAZ 8
+
+typedef struct xg_grid xg_grid;
+typedef struct xg_kernel xg_kernel;
+typedef struct xg_module xg_module;
+
+/* One stencil statement of a kernel (filled by the emitter). */
+typedef struct xg_stmt_desc {
+  int32_t store_arg;                         /* index of the store grid among the kernel's grid args */
+  int32_t klass;                             /* boundary class (mask value) the statement updates */
+  int32_t mode;                              /* XG_MODE_* */
+  int32_t dims;
+  int32_t elem_bytes;
+  int32_t nrefs;                             /* distinct (grid, level) arrays loaded */
+  int32_t ref_arg[XG_DESC_MAX_REFS];
+  int32_t ref_level[XG_DESC_MAX_REFS];       /* -1: only the grid's shape is read */
+  int32_t store_off[3];
+  int32_t nhazard;                           /* same-grid time-0 read offsets relative to the store */
+  int32_t hazard[XG_DESC_MAX_HAZ][3];
+  int32_t param_bytes;
+  int32_t overstep;                          /* 0 none, 1 limit, 2 wrap (xgrid/util/init.py:15) */
+  int32_t group;                             /* fusion group (-1: none) */
+  int32_t group_pos;
+  void* fn_main;                             /* hipFunction_t of the class kernel */
+  void* fn_pingpong;                         /* hipFunction_t of the Jacobi sweep (PINGPONG) */
+  void* fn_phase2;                           /* hipFunction_t storing a temporary (two-phase) */
+} xg_stmt_desc;
+
+/* A run of consecutive statements that may be chained in one workgroup. */
+typedef struct xg_group_desc {
+  int32_t first;
+  int32_t count;
+  void* fn;                                  /* hipFunction_t taking (uint32 enable, {hdr, refs, params} x count) */
+} xg_group_desc;
+
+/* Function table handed to generated host stubs (the stub runs the kernel's
+ * scalar control flow and issues one stmt() per executed stencil statement). */
+typedef struct xg_api {
+  int (*begin)(xg_kernel* k, xg_grid* const* grids, int32_t ngrids, const void* key, int64_t key_bytes);
+  int (*stmt)(xg_kernel* k, int32_t sid, const void* params, int64_t param_bytes);
+  int (*end)(xg_kernel* k);
+  const int32_t* (*shape)(xg_grid* g);
+} xg_api;
+
+/* ---- errors and device ------------------------------------------------- */
+/* Replaces Logger.dead exceptions (xgrid/util/logging.py:58-60). */
+const char* xg_last_error(void);
+/* Select the device and create the backend's stream.  Replaces nothing in the
+ * reference (host only); xgrid.init (xgrid/util/init.py:48-66) calls it. */
+int xg_init(int device);
+int xg_initialized(void);
+int xg_device_name(char* buf, int64_t buf_bytes, int* cus, int64_t* hbm_bytes);
+int xg_synchronize(void);
+int xg_mem_info(int64_t* free_bytes, int64_t* total_bytes);
+
+/* ---- code objects ------------------------------------------------------ */
+/* hipRTC-compile HIP source for gfx950 and write the code object to out_path.
+ * Needs no GPU.  Replaces Compiler.compile (xgrid/util/ffi.py:59-94). */
+int xg_rtc_compile(const char* src, const char* name, const char* const* opts, int32_t nopts,
+                   const char* const* hdr_names, const char* const* hdr_srcs, int32_t nhdr,
+                   const char* out_path);
+/* Load a code object and look up kernels.  Replaces Library / Library.function
+ * (xgrid/util/ffi.py:13-34). */
+int xg_module_load(const char* path, xg_module** out);
+int xg_module_function(xg_module* m, const char* name, void** out_fn);
+int xg_module_unload(xg_module* m);
+
+/* ---- grids: device-resident time-level rings --------------------------- */
+/* Storage of one Grid (xgrid/xgrid/__init__.py:22-41): shape, element size,
+ * levels allocated on demand. */
+int xg_grid_create(int32_t dims, const int32_t* shape, int32_t elem_bytes, xg_grid** out);
+int xg_grid_destroy(xg_grid* g);
+/* Grow with zero-filled levels or truncate to `depth` (Grid._extend_time, :43-47). */
+int xg_grid_set_depth(xg_grid* g, int32_t depth);
+int32_t xg_grid_depth(xg_grid* g);
+/* Rotate: the oldest level becomes level 0 (Grid._op_invoke, :49-54). */
+int xg_grid_tick(xg_grid* g);
+/* Host <-> device copies of one level (Grid.now / fill, :70-80).  Blocking. */
+int xg_grid_upload(xg_grid* g, int32_t level, const void* host);
+int xg_grid_download(xg_grid* g, int32_t level, void* host);
+/* Install the int32 boundary mask (Grid.boundary, :41) and plan each class. */
+int xg_grid_set_mask(xg_grid* g, const int32_t* mask);
+/* Plan of one class: kind (XG_DOM_*), cell count, box. */
+int xg_grid_class_info(xg_grid* g, int32_t klass, int32_t* kind, int64_t* count, int32_t* lo, int32_t* hi);
+void* xg_grid_level_ptr(xg_grid* g, int32_t level);
+
+/* ---- kernels: one compiled @xgrid.kernel ------------------------------- */
+/* Create the launch plan of a kernel from its statements.  Replaces the
+ * native handle Generator(op).result returns (xgrid/lang/operator.py:31-34). */
+int xg_kernel_create(int32_t nstmts, const xg_stmt_desc* stmts, int32_t ngroups,
+                     const xg_group_desc* groups, int32_t ngrids, xg_kernel** out);
+int xg_kernel_destroy(xg_kernel* k);
+/* Status of the last call made through the kernel's stub (0 or XG_ERR_*). */
+int xg_kernel_status(xg_kernel* k);
+/* Options: graphs (replay repeated calls as hipGraphs), fuse (chain small
+ * boundary statements in one workgroup). */
+int xg_kernel_set_options(xg_kernel* k, int32_t graphs, int32_t fuse);
+/* Launch counters and graph cache statistics. */
+int xg_kernel_counters(xg_kernel* k, int64_t* launches, int64_t* fused, int64_t* graph_replays,
+                       int64_t* graph_captures);
+/* Bracket every direct launch of statement `sid` with HIP events on the
+ * backend stream; xg_kernel_stmt_time returns the summed kernel time. */
+int xg_kernel_time_stmt(xg_kernel* k, int32_t sid, int32_t enable);
+int xg_kernel_stmt_time(xg_kernel* k, int32_t sid, double* total_ms, int64_t* launches);
+/* Executions of each statement (stmt() calls, replayed calls included) since
+ * the last reset; fills counts[0..n). */
+int xg_kernel_stmt_counts(xg_kernel* k, int64_t* counts, int32_t n, int32_t reset);
+/* The stub function table (pointers into this library). */
+const xg_api* xg_api_table(void);
+
+/* ---- measurement probes ------------------------------------------------ */
+/* Device copy of `bytes` bytes repeated `iters` times; returns GB/s (read+write). */
+int xg_copy_probe(int64_t bytes, int32_t iters, double* gbps, double* ms_per_iter);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

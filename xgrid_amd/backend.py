@@ -1,0 +1,208 @@
+"""Build and call a kernel on the MI355X backend.
+
+``build_kernel(op)`` returns ``(native, depth)`` - the pair the reference's
+``Generator(op).result`` returns (``xgrid/lang/generator.py:106-108``) - where
+``native`` is a callable with the reference's calling convention
+(``xgrid/util/ffi.py:30-34``: positional arguments, arity checked, scalars and
+dataclasses converted to C, the C return value converted back).  Underneath:
+hipRTC code object + host stub + ``xg_kernel`` launch plan.  There is no CPU
+fallback: without the library or a GPU this raises.
+"""
+from __future__ import annotations
+
+import ctypes
+from ctypes import c_void_p
+
+from . import _lib
+from .analysis import analyze
+from .compilers import compile_device, compile_host, load_host
+from .config import get_config
+from .emit import Emitter, Lowered
+from .grid import Grid
+from .types import Boolean, Floating, GridType, Integer, Pointer, Structure, Void
+
+_MODE = {"inplace": _lib.MODE_INPLACE, "pingpong": _lib.MODE_PINGPONG, "twophase": _lib.MODE_TWOPHASE}
+
+
+class KernelError(RuntimeError):
+    pass
+
+
+def lower_kernel(op) -> Lowered:
+    cfg = get_config()
+    info = analyze(op.ir)
+    if not cfg.parallel and any(st.implicit for st in info.stmts):
+        raise KernelError("Unable to perform implicit operation while parallel is off")
+    return Emitter(info, cfg.overstep, cfg.comment, macro=op.macro).lower(), info
+
+
+def _ctype_of(t):
+    if isinstance(t, GridType):
+        return c_void_p
+    if isinstance(t, Pointer):
+        return ctypes.POINTER(t.element.ctype)
+    return t.ctype
+
+
+class HipKernel:
+    """A compiled kernel: code object, stub entry point and launch plan."""
+
+    def __init__(self, op):
+        cfg = get_config()
+        self.op = op
+        self.lib = _lib.ensure_device(cfg.device)
+        low, info = lower_kernel(op)
+        self.lowered = low
+        self.info = info
+        self.depth = info.depth
+        self.code_object = compile_device(low.device_src, cfg.cacheroot, cfg.fp_contract)
+        self.stub_path = compile_host(low.stub_src, cfg.cacheroot, cfg.cc, cfg.opt_level)
+        mod = c_void_p()
+        _lib.check(self.lib.xg_module_load(self.code_object.encode(), ctypes.byref(mod)), "xg_module_load")
+        self.module = mod.value
+
+        def fn(name):
+            if name is None:
+                return None
+            out = c_void_p()
+            _lib.check(self.lib.xg_module_function(self.module, name.encode(), ctypes.byref(out)),
+                       f"kernel {name}")
+            return out.value
+
+        grid_index = {n: k for k, (_, n, _) in enumerate(low.grid_args)}
+        self.grid_index = grid_index
+        descs = (_lib.StmtDesc * max(1, len(low.stmts)))()
+        overstep = {"none": 0, "limit": 1, "wrap": 2}[cfg.overstep]
+        self.param_bytes = []
+        for i, m in enumerate(low.stmts):
+            st = m.stmt
+            d = descs[i]
+            d.store_arg = grid_index[st.grid]
+            d.klass = st.klass
+            d.mode = _MODE[st.mode]
+            d.dims = st.dimension
+            d.elem_bytes = st.gtype.element.nbytes if hasattr(st.gtype.element, "nbytes") else 1
+            d.nrefs = len(m.refs)
+            for r, (g, lvl) in enumerate(m.refs):
+                d.ref_arg[r] = grid_index[g]
+                d.ref_level[r] = lvl
+            so = list(st.store_offsets) + [0] * (3 - len(st.store_offsets))
+            for a in range(3):
+                d.store_off[a] = so[a]
+            haz = st.hazard_offsets if st.mode == "inplace" else []
+            if len(haz) > _lib.XG_MAX_HAZ:
+                haz = []                       # too many to check per cell: always compute-then-store
+                d.mode = _lib.MODE_TWOPHASE
+            d.nhazard = len(haz)
+            for h, off in enumerate(haz):
+                off3 = list(off) + [0] * (3 - len(off))
+                for a in range(3):
+                    d.hazard[h][a] = off3[a]
+            d.param_bytes = self._param_size(m)
+            d.overstep = overstep
+            d.group = m.group
+            d.group_pos = m.group_pos
+            d.fn_main = fn(m.kernel)
+            d.fn_pingpong = fn(m.kernel_pp)
+            d.fn_phase2 = fn(m.phase2)
+        groups = (_lib.GroupDesc * max(1, len(low.groups)))()
+        for i, g in enumerate(low.groups):
+            groups[i].first = g.first
+            groups[i].count = g.count
+            groups[i].fn = fn(g.kernel)
+        k = c_void_p()
+        _lib.check(self.lib.xg_kernel_create(len(low.stmts), descs, len(low.groups), groups, len(low.grid_args),
+                                             ctypes.byref(k)), "xg_kernel_create")
+        self.kernel = k.value
+        _lib.check(self.lib.xg_kernel_set_options(self.kernel, int(cfg.graphs and low.graphs_ok), int(cfg.fuse)))
+        self.api = self.lib.xg_api_table()
+        self.stub = load_host(self.stub_path)
+        self.entry = self.stub.xg_entry
+        self.arg_types = list(op.ir.args)
+        self.entry.argtypes = [c_void_p, c_void_p] + [_ctype_of(t) for _, t in self.arg_types]
+        ret = op.ir.ret
+        self.ret = ret
+        self.entry.restype = None if isinstance(ret, Void) else ret.ctype
+        self.written = [grid_index[g] for g in info.written]
+
+    def _param_size(self, m) -> int:
+        fields_ = [(v.name, (v.type.element if isinstance(v.type, Pointer) else v.type).ctype) for v in m.params]
+        if not fields_:
+            fields_ = [("xg_unused", ctypes.c_int32)]
+        return ctypes.sizeof(type("P", (ctypes.Structure,), {"_fields_": fields_}))
+
+    def set_options(self, graphs: bool | None = None, fuse: bool | None = None):
+        cfg = get_config()
+        g = cfg.graphs and self.lowered.graphs_ok if graphs is None else graphs
+        f = cfg.fuse if fuse is None else fuse
+        _lib.check(self.lib.xg_kernel_set_options(self.kernel, int(g), int(f)))
+
+    def counters(self) -> dict:
+        vals = [ctypes.c_int64() for _ in range(4)]
+        _lib.check(self.lib.xg_kernel_counters(self.kernel, *[ctypes.byref(v) for v in vals]))
+        return dict(zip(("launches", "fused", "graph_replays", "graph_captures"), (v.value for v in vals)))
+
+    def statement_counts(self, reset: bool = False) -> list:
+        n = len(self.lowered.stmts)
+        arr = (ctypes.c_int64 * max(1, n))()
+        _lib.check(self.lib.xg_kernel_stmt_counts(self.kernel, arr, n, int(reset)))
+        return list(arr)[:n]
+
+    def algorithmic_bytes(self, counts: list, grids: list) -> int:
+        """Sum over executed statements of cells(class) x element bytes x (arrays loaded + 1)."""
+        total = 0
+        for m, c in zip(self.lowered.stmts, counts):
+            if not c:
+                continue
+            st = m.stmt
+            g = grids[self.grid_index[st.grid]]
+            cells = g.class_info(st.klass)["count"]
+            loaded = len([r for r in m.refs if r[1] >= 0])
+            total += c * cells * g.numpy_dtype.itemsize * (loaded + 1)
+        return total
+
+    def time_statement(self, sid: int, enable: bool = True):
+        _lib.check(self.lib.xg_kernel_time_stmt(self.kernel, sid, int(enable)))
+
+    def statement_time(self, sid: int):
+        ms, n = ctypes.c_double(), ctypes.c_int64()
+        _lib.check(self.lib.xg_kernel_stmt_time(self.kernel, sid, ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
+
+    def __call__(self, *args):
+        if len(args) != len(self.arg_types):
+            raise TypeError(f"this function takes {len(self.arg_types)} argument ({len(args)} given)")
+        cargs = [self.kernel, self.api]
+        grids = []
+        for (name, t), a in zip(self.arg_types, args):
+            if isinstance(t, GridType):
+                if not isinstance(a, Grid):
+                    raise TypeError(f"argument '{name}' expects a Grid")
+                if a.dimension != t.dimension or a.element != t.element:
+                    raise TypeError(f"argument '{name}' expects {t!r}, got a grid of {a.element!r} with "
+                                    f"{a.dimension} dimension(s)")
+                cargs.append(a._sync_for_launch())
+                grids.append(a)
+            elif isinstance(t, Structure):
+                cargs.append(t.to_c(a))
+            elif isinstance(t, Pointer):
+                if isinstance(a, ctypes._Pointer):
+                    cargs.append(a)
+                else:
+                    cargs.append(ctypes.pointer(a))
+            else:
+                cargs.append(a)
+        result = self.entry(*cargs)
+        rc = self.lib.xg_kernel_status(self.kernel)
+        for i in self.written:
+            grids[i]._device_wrote()
+        if rc != 0:
+            raise _lib.BackendError(f"kernel '{self.op.name}' failed: {_lib.last_error()} (code {rc})")
+        if isinstance(self.ret, Structure):
+            return self.ret.from_c(result)
+        return result
+
+
+def build_kernel(op):
+    k = HipKernel(op)
+    return k, k.depth
