@@ -135,6 +135,10 @@ int xg_kernel_counters(xg_kernel* k, int64_t* launches, int64_t* fused, int64_t*
  * backend stream; xg_kernel_stmt_time returns the summed kernel time. */
 int xg_kernel_time_stmt(xg_kernel* k, int32_t sid, int32_t enable);
 int xg_kernel_stmt_time(xg_kernel* k, int32_t sid, double* total_ms, int64_t* launches);
+/* Launch statement `sid` `reps` times back to back on the grids and with the
+ * parameters of its last call, between two HIP events; returns ms per launch.
+ * Advances the grids' state like that many extra executions would. */
+int xg_kernel_time_isolated(xg_kernel* k, int32_t sid, int32_t reps, double* ms_per_launch);
 /* Executions of each statement (stmt() calls, replayed calls included) since
  * the last reset; fills counts[0..n). */
 int xg_kernel_stmt_counts(xg_kernel* k, int64_t* counts, int32_t n, int32_t reset);

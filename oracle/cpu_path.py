@@ -345,6 +345,7 @@ class CSource:
         text = "\n".join(out) + "\n"
         if not export:
             self.funcs[sym] = text
+            return sym
         return text
 
     def source(self) -> str:

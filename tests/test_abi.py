@@ -1,0 +1,106 @@
+"""The C-ABI library (no GPU): it loads, exports every symbol include/xgrid_hip.h
+declares, its structs have one layout in C and in ctypes, generated kernels
+compile for gfx950 with hipRTC, and the device path fails loudly without a GPU."""
+import ctypes
+import os
+import re
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+from xgrid_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "xgrid_hip.h")
+
+
+def declared_symbols():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"^\s*(?:const\s+)?[a-z_0-9]+\s*\**\s*\b(xg_[a-z_0-9]+)\s*\(", text, flags=re.M)
+    return sorted(set(names))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.load()
+    names = declared_symbols()
+    assert len(names) >= 30
+    for n in names:
+        assert hasattr(lib, n), f"missing export {n}"
+    assert set(names) == set(_lib.EXPORTED), set(names) ^ set(_lib.EXPORTED)
+
+
+def test_struct_layouts_match_c():
+    src = r'''
+#include <stdio.h>
+#include <stddef.h>
+#include "xgrid_hip.h"
+#include "xgrid_abi.h"
+int main(void) {
+  printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(xg_stmt_desc), offsetof(xg_stmt_desc, param_bytes),
+         offsetof(xg_stmt_desc, fn_main), sizeof(xg_group_desc), sizeof(xg_api), sizeof(xg_dom),
+         sizeof(xg_gview), sizeof(xg_launch_hdr), offsetof(xg_dom, count));
+  return 0;
+}
+'''
+    with tempfile.TemporaryDirectory() as d:
+        c = os.path.join(d, "l.c")
+        open(c, "w").write(src)
+        exe = os.path.join(d, "l")
+        subprocess.run(["gcc", c, "-I", os.path.join(ROOT, "include"), "-I", os.path.join(ROOT, "xgrid_amd", "csrc"),
+                        "-o", exe], check=True)
+        got = [int(x) for x in subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split()]
+    exp = [ctypes.sizeof(_lib.StmtDesc), _lib.StmtDesc.param_bytes.offset, _lib.StmtDesc.fn_main.offset,
+           ctypes.sizeof(_lib.GroupDesc), ctypes.sizeof(_lib.Api), 72, 24, 120, 32]
+    assert got == exp
+
+
+def test_device_path_fails_loudly_without_gpu(double):
+    if _lib.load().xg_initialized():
+        pytest.skip("a device is initialised")
+    lib = _lib.load()
+    n = ctypes.c_int()
+    if lib.xg_init(0) == 0:
+        pytest.skip("a GPU is present")
+    from xgrid_amd import workloads as W
+    k = W.make_diffusion_1d()
+    g = __import__("xgrid_amd").Grid((16,), float)
+    with pytest.raises(_lib.BackendError):
+        k(g, 0.1, 0.1, 0.1)
+
+
+def test_calls_before_init_are_refused():
+    lib = _lib.load()
+    if lib.xg_initialized():
+        pytest.skip("device initialised")
+    h = ctypes.c_void_p()
+    shape = (ctypes.c_int32 * 3)(4, 1, 1)
+    assert lib.xg_grid_create(1, shape, 8, ctypes.byref(h)) == -5
+    assert "xg_init" in _lib.last_error()
+
+
+@pytest.mark.parametrize("prec", ["double", "float"])
+def test_generated_kernels_compile_for_gfx950(prec, request, tmp_path):
+    request.getfixturevalue(prec if prec == "double" else "single")
+    from xgrid_amd import workloads as W
+    from xgrid_amd.backend import lower_kernel
+    from xgrid_amd.compilers import compile_device, compile_host
+    for make in (W.make_diffusion_1d, W.make_cavity, W.make_heat_3d, W.make_convection_2d):
+        low, _ = lower_kernel(make())
+        assert os.path.getsize(compile_device(low.device_src, str(tmp_path))) > 1000
+        assert os.path.exists(compile_host(low.stub_src, str(tmp_path), ["gcc"]))
+
+
+def test_rtc_compile_error_is_reported(tmp_path):
+    from xgrid_amd.compilers import CompileError, compile_device
+    with pytest.raises(CompileError) as e:
+        compile_device('extern "C" __global__ void k() { undeclared_thing(); }', str(tmp_path))
+    assert "undeclared_thing" in str(e.value)
+
+
+def test_host_compile_error_is_reported(tmp_path):
+    from xgrid_amd.compilers import CompileError, compile_host
+    with pytest.raises(CompileError):
+        compile_host("innt main() {}", str(tmp_path), ["gcc"])

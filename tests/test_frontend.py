@@ -1,0 +1,192 @@
+"""Frontend semantics (no GPU): the rules of xgrid/lang/parser.py and the
+stencil analysis of xgrid/lang/generator.py:50-76."""
+import dataclasses
+
+import numpy as np
+import pytest
+
+import xgrid_amd as xgrid
+from xgrid_amd import ir
+from xgrid_amd.analysis import StencilError, analyze
+from xgrid_amd.frontend import DSLSyntaxError
+from xgrid_amd.types import Boolean, Floating, GridType, Integer, Pointer, Structure, parse_annotation
+
+f1 = xgrid.grid[float, 1]
+f2 = xgrid.grid[float, 2]
+CONST = 7
+
+
+def test_annotations(double):
+    assert parse_annotation(int) == Integer(4)
+    assert parse_annotation(float) == Floating(8)
+    assert parse_annotation(bool) == Boolean()
+    assert parse_annotation(xgrid.grid[float, 3]) == GridType(Floating(8), 3)
+    assert parse_annotation(xgrid.ptr[int]) == Pointer(Integer(4))
+    xgrid.init(precision="float")
+    assert parse_annotation(float) == Floating(4)
+
+
+def test_time_offsets_default_load_minus1_store_0(double):
+    @xgrid.kernel()
+    def k(u: f1, v: f1) -> None:
+        u[0] = v[1] + v[-1][0] + u[0][-2]
+
+    f = k.ir
+    st = f.body[0]
+    assert isinstance(st.target, ir.Cell) and st.target.time == 0 and st.target.ctx == "store"
+    cells = [n for n in ir.walk(st.value) if isinstance(n, ir.Cell)]
+    assert [(c.grid.name, c.offsets, c.time) for c in cells] == [("v", (1,), -1), ("v", (-1,), 0), ("u", (0,), -2)]
+    info = analyze(f)
+    assert info.depth == 3                      # max |t| + 1
+    assert info.stmts[0].refs == [("v", 1), ("v", 0), ("u", 2)]
+
+
+def test_implicit_rule(double):
+    @xgrid.kernel()
+    def k(p: f2, b: f2) -> None:
+        p[0, 0] = p[0, 1][0] + b[0, 0]          # self read at time 0, class 0: implicit
+        p[0, 0] = p[0, 1] + b[0, 0][0]          # time -1 self read: explicit
+        with xgrid.boundary(2):
+            p[0, 0] = p[1, 0][0]                # class 2: in place (not implicit)
+        p[0, 0] = p[0, 0][0] * 2.0              # only the cell itself: implicit but in place
+
+    info = analyze(k.ir)
+    assert [s.implicit for s in info.stmts] == [True, False, False, True]
+    assert [s.mode for s in info.stmts] == ["pingpong", "inplace", "inplace", "inplace"]
+    assert [s.klass for s in info.stmts] == [0, 0, 2, 0]
+    assert info.stmts[2].hazard_offsets == [(1, 0)]
+
+
+def test_store_offset_implicit_is_twophase(double):
+    @xgrid.kernel()
+    def k(p: f1) -> None:
+        p[1] = p[-1][0]
+
+    assert analyze(k.ir).stmts[0].mode == "twophase"
+
+
+def test_chained_compare_compares_left_with_each(double):
+    @xgrid.kernel()
+    def k(a: int, b: int, c: int) -> bool:
+        return a < b < c
+
+    ret = k.ir.body[0].value
+    assert ret.op == "&&"
+    assert ret.left.left.var.name == "a" and ret.right.left.var.name == "a"   # a<b && a<c
+
+
+def test_pow_typing(double):
+    @xgrid.kernel()
+    def k(x: float, i: int) -> float:
+        y = x ** 2.0
+        z = i ** 2
+        return y
+
+    body = k.ir.body
+    assert body[0].value.type == Floating(8)
+    assert body[1].value.type == Floating(8)        # int ** int -> configured float
+    xgrid.init(precision="float")
+
+    @xgrid.kernel()
+    def k2(x: float) -> float:
+        return x ** 2.0
+
+    assert k2.ir.body[0].value.type == Floating(4)
+
+
+def test_globals_are_folded(double):
+    @xgrid.kernel()
+    def k(a: int) -> int:
+        return a + CONST
+
+    c = k.ir.body[0].value.right
+    assert isinstance(c, ir.Const) and c.value == 7
+
+
+@pytest.mark.parametrize("src_case", ["mixed", "unknown", "load_outside", "two_arg_boundary", "bad_offset",
+                                      "grid_value", "pow_aug"])
+def test_syntax_errors(double, src_case):
+    if src_case == "mixed":
+        @xgrid.kernel()
+        def k(a: int, b: float) -> float:
+            return a + b
+    elif src_case == "unknown":
+        @xgrid.kernel()
+        def k(a: int) -> int:
+            return a + not_defined_anywhere  # noqa: F821
+    elif src_case == "load_outside":
+        @xgrid.kernel()
+        def k(u: f1) -> float:
+            return u[0]
+    elif src_case == "two_arg_boundary":
+        @xgrid.kernel()
+        def k(u: f1) -> None:
+            with xgrid.boundary(u, 1):
+                u[0] = 1.0
+    elif src_case == "bad_offset":
+        @xgrid.kernel()
+        def k(u: f1, i: int) -> None:
+            u[i] = 1.0
+    elif src_case == "grid_value":
+        @xgrid.kernel()
+        def k(u: f1) -> None:
+            x = u
+    else:
+        @xgrid.kernel()
+        def k(x: float) -> float:
+            x **= 2.0
+            return x
+    with pytest.raises((DSLSyntaxError, StencilError)):
+        analyze(k.ir)
+
+
+def test_struct_method_and_constructor(double):
+    @dataclasses.dataclass
+    class V2:
+        x: float
+        y: float
+
+        @xgrid.function(method=True)
+        def norm2(self) -> float:
+            return self.x * self.x + self.y * self.y
+
+    @xgrid.kernel()
+    def k(a: V2) -> float:
+        b = V2(a.y, a.x)
+        return b.norm2()
+
+    body = k.ir.body
+    assert isinstance(body[0].value, ir.Call) and isinstance(body[0].value.ctor, Structure)
+    assert body[1].value.callee.c_name == "V2_norm2"
+
+
+def test_readme_elementwise_semantics_on_host_grids(double):
+    """README.md:26-57: every Grid argument is ticked, so after one call the
+    inputs' `now` is a fresh zero level and `result` holds a*b."""
+    from oracle.cpu_path import CpuKernel, HostGrid
+
+    @xgrid.kernel()
+    def elementwise_mul(result: f1, a: f1, b: f1) -> None:
+        result[0] = a[0] * b[0]
+
+    rng = np.random.default_rng(0)
+    r, a, b = HostGrid((100,), float), HostGrid((100,), float), HostGrid((100,), float)
+    av, bv = rng.random(100), rng.random(100)
+    a.now[:] = av
+    b.now[:] = bv
+    ck = CpuKernel(elementwise_mul)
+    ck(r, a, b)
+    assert np.array_equal(r.now, av * bv)
+    assert not a.now.any() and not b.now.any()
+    ck(r, a, b)                                  # second call reads the zero levels
+    assert not r.now.any()
+
+
+def test_fill_default_time_truncates_and_raises(double):
+    """xgrid/xgrid/__init__.py:74-80: fill(data) with time=0 empties the level list."""
+    g = xgrid.Grid((4,), float)
+    with pytest.raises(IndexError):
+        g.fill(np.ones(4))
+    g2 = xgrid.Grid((4,), float)
+    g2.fill(np.full(4, 3.0), time=-1)
+    assert (g2.now == 3.0).all()

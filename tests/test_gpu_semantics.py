@@ -1,0 +1,319 @@
+"""Device semantics vs the oracle: time levels, ticking, masks, overstep, the
+implicit / in-place rules, aliasing, host writes between calls, graph replay
+and the reference's known answers (test.py)."""
+import ctypes
+import dataclasses
+import random
+
+import numpy as np
+import pytest
+
+import xgrid_amd as xgrid
+from oracle.cpu_path import CpuKernel, HostGrid
+from tests.helpers import assert_bitexact, grid_pair, run_pair
+
+pytestmark = pytest.mark.gpu
+
+f1 = xgrid.grid[float, 1]
+f2 = xgrid.grid[float, 2]
+f3 = xgrid.grid[float, 3]
+i2 = xgrid.grid[int, 2]
+TEMP = 10
+
+
+@dataclasses.dataclass
+class Vector3i:
+    x: int
+    y: int
+    z: int
+
+    @xgrid.function(method=True)
+    def dot(self, b: "Vector3i") -> int:
+        return self.x * b.x + self.y * b.y + self.z * b.z
+
+
+def test_scalar_and_struct_kernels(double):
+    @xgrid.kernel()
+    def aux(a: int, b: int) -> int:
+        return a + b + TEMP
+
+    @xgrid.kernel()
+    def aux2(a: Vector3i, b: Vector3i) -> int:
+        return a.dot(b)
+
+    for _ in range(5):
+        a, b = random.randint(0, 1000), random.randint(0, 1000)
+        assert aux(a, b) == a + b + TEMP
+        va = Vector3i(*(random.randint(0, 1000) for _ in range(3)))
+        vb = Vector3i(*(random.randint(0, 1000) for _ in range(3)))
+        assert aux2(va, vb) == va.dot(vb)
+    with pytest.raises(TypeError):
+        aux(1)
+
+
+def test_float_add_known_answer(single):
+    @xgrid.kernel()
+    def universe(a: float, b: float) -> float:
+        return a + b
+
+    assert universe(1.2, 2.3) == 3.5
+
+
+def test_int_grid_fill(double):
+    @xgrid.kernel()
+    def aux(a: i2) -> None:
+        a[0, 0] = 4
+
+    g = xgrid.Grid((10, 10), dtype=int)
+    aux(g)
+    assert g.now.dtype == np.int32 and (g.now == 4).all()
+
+
+def test_index_guard_runs_and_matches_inside_the_array(double):
+    @xgrid.kernel()
+    def aux(a: i2) -> None:
+        a[0, 0] = a[-1, -1][-1]
+
+    rng = np.random.default_rng(1)
+    init = rng.integers(0, 100, (10, 10), dtype=np.int32)
+    d, h = grid_pair(xgrid, (10, 10), int, None, init)
+    run_pair(aux, (d,), (h,), steps=3)
+    # cells 0..10 read before element 0 (undefined in the reference); each step
+    # carries those values 11 cells further
+    dv, hv = d.now.reshape(-1), h.now.reshape(-1)
+    assert np.array_equal(dv[33:], hv[33:])
+
+
+def test_readme_elementwise_ticks_every_grid(double):
+    @xgrid.kernel()
+    def elementwise_mul(result: f1, a: f1, b: f1) -> None:
+        result[0] = a[0] * b[0]
+
+    rng = np.random.default_rng(0)
+    av, bv = rng.random(1000), rng.random(1000)
+    r = xgrid.Grid((1000,), float)
+    a = xgrid.Grid((1000,), float)
+    b = xgrid.Grid((1000,), float)
+    a.now[:] = av
+    b.now[:] = bv
+    elementwise_mul(r, a, b)
+    assert np.array_equal(r.now, av * bv)
+    assert not a.now.any() and not b.now.any()
+    elementwise_mul(r, a, b)
+    assert not r.now.any()
+
+
+def test_three_time_levels(double):
+    @xgrid.kernel()
+    def leapfrog(u: f1, c: float) -> None:
+        u[0] = u[0][-2] + c * (u[1] - u[-1])
+        with xgrid.boundary(1):
+            u[0] = 0.0
+
+    rng = np.random.default_rng(2)
+    mask = np.zeros(64, np.int32)
+    mask[0] = mask[-1] = 1
+    d, h = grid_pair(xgrid, (64,), float, mask, rng.random(64))
+    run_pair(leapfrog, (d, 0.3), (h, 0.3), steps=9)
+    assert d.time_levels == 3
+    assert_bitexact(d, h)
+
+
+@pytest.mark.parametrize("shape", [(37, 53), (8, 9, 11)])
+def test_non_square_grids_rowmajor(double, shape):
+    from xgrid_amd import workloads as W
+    rng = np.random.default_rng(3)
+    if len(shape) == 2:
+        k = W.make_diffusion_2d()
+    else:
+        k = W.make_heat_3d()
+    d, h = grid_pair(xgrid, shape, float, W.box_walls(shape), rng.random(shape))
+    run_pair(k, (d, 0.05, 1e-4, 0.02), (h, 0.05, 1e-4, 0.02), steps=5)
+    assert_bitexact(d, h)
+
+
+@pytest.mark.parametrize("mode", ["wrap", "limit"])
+def test_overstep_modes(mode):
+    xgrid.init(precision="double", overstep=mode)
+
+    @xgrid.kernel()
+    def smooth(u: f2, c: float) -> None:
+        u[0, 0] = u[0, 0] + c * (u[1, 0] + u[-1, 0] + u[0, 1] + u[0, -2] - 4.0 * u[0, 0])
+
+    rng = np.random.default_rng(4)
+    d, h = grid_pair(xgrid, (24, 24), float, None, rng.random((24, 24)))
+    run_pair(smooth, (d, 0.1), (h, 0.1), steps=6)
+    assert_bitexact(d, h)
+    if mode == "wrap":
+        assert abs(d.now.sum() - h.now.sum()) < 1e-9
+
+
+def test_twophase_store_offset(double):
+    @xgrid.kernel()
+    def shift(u: f1) -> None:
+        u[1] = u[-1][0] * 2.0
+
+    rng = np.random.default_rng(5)
+    mask = np.zeros(40, np.int32)
+    mask[0] = mask[-1] = 7              # cell 0 would read before the array
+    d, h = grid_pair(xgrid, (40,), float, mask, rng.random(40))
+    run_pair(shift, (d,), (h,), steps=4)
+    # cells 1..38 are stored from cells 0..37 of the same level (reads before writes)
+    assert_bitexact(d, h)
+
+
+def test_same_class_reads_run_compute_then_store(double):
+    """A boundary statement reading its own class at time 0 is a race in the
+    reference's parallel C; the device runs it reading the old values."""
+    @xgrid.kernel()
+    def chain(u: f2) -> None:
+        with xgrid.boundary(1):
+            u[0, 0] = u[0, -1][0] + 1.0
+
+    mask = np.zeros((6, 10), np.int32)
+    mask[2, :] = 1
+    init = np.arange(60, dtype=np.float64).reshape(6, 10)
+    g = xgrid.Grid((6, 10), float)
+    g.now[...] = init
+    g.boundary[...] = mask
+    chain(g)                            # depth 1: level 0 is the initial data
+    exp = init.copy()
+    flat_old = init.reshape(-1)
+    for j in range(10):
+        exp[2, j] = flat_old[2 * 10 + j - 1] + 1.0
+    assert np.array_equal(g.now, exp)
+
+
+def test_aliased_grid_argument(double):
+    @xgrid.kernel()
+    def two(a: f1, b: f1) -> None:
+        a[0] = b[1] + 1.0
+
+    rng = np.random.default_rng(6)
+    init = rng.random(30)
+    d = xgrid.Grid((30,), float)
+    h = HostGrid((30,), float)
+    d.now[:] = init
+    h.now[:] = init
+    d.boundary[-1] = h.boundary[-1] = 1
+    ck = CpuKernel(two)
+    for _ in range(3):
+        two(d, d)
+        ck(h, h)
+    assert_bitexact(d, h)
+
+
+def test_host_writes_and_mask_changes_between_calls(double):
+    from xgrid_amd import workloads as W
+    k = W.make_diffusion_2d()
+    rng = np.random.default_rng(7)
+    d, h = grid_pair(xgrid, (33, 33), float, W.box_walls((33, 33)), rng.random((33, 33)))
+    ck = CpuKernel(k)
+    for step in range(8):
+        if step == 3:
+            v = rng.random((33, 33))
+            d.now[...] = v
+            h.now[...] = v
+        if step == 5:
+            m = W.box_walls((33, 33))
+            m[10:14, 10:20] = 1        # an obstacle
+            d.boundary[...] = m
+            h.boundary[...] = m
+        k(d, 0.05, 1e-4, 0.02)
+        ck(h, 0.05, 1e-4, 0.02)
+    assert_bitexact(d, h)
+
+
+def test_list_and_masked_classes(double):
+    @xgrid.kernel()
+    def k(u: f2, v: f2) -> None:
+        u[0, 0] = v[0, 1] + v[-1, 0] * 0.5
+        with xgrid.boundary(3):
+            u[0, 0] = v[0, 0] * 2.0 - 1.0
+        with xgrid.boundary(5):
+            u[0, 0] = u[0, 0][0] + v[1, 1]
+
+    rng = np.random.default_rng(8)
+    n = 50
+    mask = np.where(rng.random((n, n)) < 0.3, 0, 9).astype(np.int32)  # class 0: irregular (masked)
+    mask[0, :] = mask[-1, :] = mask[:, 0] = mask[:, -1] = 9           # keep reads inside the array
+    inner = [i * n + j for i in range(1, n - 1) for j in range(1, n - 1)]
+    idx = rng.choice(inner, 40, replace=False)
+    mask.reshape(-1)[idx[:20]] = 3                                     # sparse: list
+    mask.reshape(-1)[idx[20:]] = 5
+    du, hu = grid_pair(xgrid, (n, n), float, mask, rng.random((n, n)))
+    dv, hv = grid_pair(xgrid, (n, n), float, None, rng.random((n, n)))
+    run_pair(k, (du, dv), (hu, hv), steps=4)
+    assert du.class_info(3)["kind"] == "list" and du.class_info(0)["kind"] in ("masked", "list")
+    assert_bitexact(du, hu)
+    assert_bitexact(dv, hv)
+
+
+def test_float_precision_mixed_literals(single):
+    """precision='float': literals are C doubles, so arithmetic is mixed (F11)."""
+    @xgrid.kernel()
+    def k(u: f1, c: float) -> None:
+        u[0] = u[0] * 0.1 + c * u[1] / 3.0
+
+    rng = np.random.default_rng(9)
+    mask = np.zeros(100, np.int32)
+    mask[-1] = 1
+    d, h = grid_pair(xgrid, (100,), float, mask, rng.random(100).astype(np.float32))
+    run_pair(k, (d, np.float32(0.3).item()), (h, np.float32(0.3).item()), steps=5)
+    assert_bitexact(d, h)
+
+
+def test_graphs_and_fusion_do_not_change_results(double):
+    from xgrid_amd import workloads as W
+    n = 65
+    cfg = W.cavity_config(n, dt=2e-6)
+    masks = W.cavity_masks(n)
+    out = []
+    for graphs, fuse in [(True, True), (False, False), (True, False)]:
+        xgrid.init(precision="double", graphs=graphs, fuse=fuse)
+        k = W.make_cavity()
+        gs = [xgrid.Grid((n, n), float) for _ in "bpuv"]
+        for g, c in zip(gs, "bpuv"):
+            g.boundary[...] = masks[c]
+        for _ in range(6):
+            k(*gs, cfg)
+        out.append([g.now.copy() for g in gs])
+        if graphs:
+            assert k.native.counters()["graph_replays"] >= 2
+    for other in out[1:]:
+        for a, b in zip(out[0], other):
+            assert np.array_equal(a, b)
+
+
+def test_pointer_argument(double):
+    @xgrid.kernel()
+    def k(u: f1, s: xgrid.ptr[float]) -> None:
+        u[0] = u[1] * s
+
+    rng = np.random.default_rng(10)
+    mask = np.zeros(20, np.int32)
+    mask[-1] = 1
+    d, h = grid_pair(xgrid, (20,), float, mask, rng.random(20))
+    s1, s2 = ctypes.c_double(1.5), ctypes.c_double(1.5)
+    run_pair(k, (d, s1), (h, s2), steps=2)
+    assert_bitexact(d, h)
+
+
+def test_control_flow_and_shape_intrinsics(double):
+    @xgrid.kernel()
+    def k(u: f2, n: int) -> int:
+        total = 0
+        for i in range(0, n):
+            if i % 2 == 0:
+                u[0, 0] = u[0, 0][0] + 1.0
+            else:
+                u[0, 0] = u[0, 0][0] * 0.5
+            total = total + xgrid.shape(u, 1)
+        while total > 100:
+            total = total - 7
+        return total + xgrid.dimension(u)
+
+    rng = np.random.default_rng(11)
+    d, h = grid_pair(xgrid, (5, 12), float, None, rng.random((5, 12)))
+    run_pair(k, (d, 5), (h, 5), steps=2)
+    assert_bitexact(d, h)

@@ -76,6 +76,7 @@ _SIGS = {
     "xg_kernel_time_stmt": ([c_void_p, c_int32, c_int32], c_int),
     "xg_kernel_stmt_time": ([c_void_p, c_int32, POINTER(c_double), POINTER(c_int64)], c_int),
     "xg_kernel_stmt_counts": ([c_void_p, POINTER(c_int64), c_int32, c_int32], c_int),
+    "xg_kernel_time_isolated": ([c_void_p, c_int32, c_int32, POINTER(c_double)], c_int),
     "xg_api_table": ([], c_void_p),
     "xg_copy_probe": ([c_int64, c_int32, POINTER(c_double), POINTER(c_double)], c_int),
 }

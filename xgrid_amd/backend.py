@@ -161,6 +161,12 @@ class HipKernel:
             total += c * cells * g.numpy_dtype.itemsize * (loaded + 1)
         return total
 
+    def time_isolated(self, sid: int, reps: int = 100) -> float:
+        """ms per launch of statement `sid`, launched back to back with its last parameters."""
+        ms = ctypes.c_double()
+        _lib.check(self.lib.xg_kernel_time_isolated(self.kernel, sid, reps, ctypes.byref(ms)))
+        return ms.value
+
     def time_statement(self, sid: int, enable: bool = True):
         _lib.check(self.lib.xg_kernel_time_stmt(self.kernel, sid, int(enable)))
 

@@ -1,8 +1,10 @@
 /* Device templates for generated stencil statements (gfx950, compiled by hipRTC).
  *
  * A generated statement is a struct S with
- *   typedef <elem> T;  static constexpr int D;  struct Refs; struct Params;
- *   static T value(const Refs&, const Params&, int i0, int i1, int i2);
+ *   typedef <elem> T;  static constexpr int D, SO0..2 (store offset),
+ *   RL0..2 / RH0..2 (largest backward / forward load offset per dimension);
+ *   struct Refs; struct Params;
+ *   template <bool SAFE> static T value(const Refs&, const Params&, int i0, int i1, int i2);
  * and the templates below decide which cells evaluate it:
  *   run_main      - the cells of one boundary class (box / list / masked box),
  *                   stored in place (reference nest, xgrid/lang/generator.py:289-304,353-361)
@@ -11,29 +13,40 @@
  *                   the old level, other cells copied (replaces the two-phase nest
  *                   with its per-sweep malloc and copy-back, generator.py:312-352);
  *   run_phase2    - phase 2 of a two-phase statement: temporary -> store cells;
- *   run_block     - one workgroup walks a small class (fused boundary chains).
+ *   run_block     - one workgroup walks a small class (chained boundary statements).
  * Linear indices are row-major; for 1-D and square 2-D grids they coincide with
  * the reference's accessor (generator.py:158-182).
+ *
+ * Each work-item owns NC cells of a workgroup tile.  It evaluates all of them
+ * before storing any (the statement never reads a cell it writes: the runtime
+ * sends statements that could to the two-phase path), so its loads are all in
+ * flight together.  Tiles whose cells and stencil reach lie inside the grid
+ * (the common case) read without the per-load clamp/wrap.
  */
 #ifndef XGRID_DEVICE_H
 #define XGRID_DEVICE_H
 
+#if !defined(__HIPCC_RTC__)
+#include <hip/hip_runtime.h>  /* offline builds (hipcc); hipRTC predefines the same */
+#endif
 #include "xgrid_abi.h"
 
 #define XG_OV_NONE 0
 #define XG_OV_LIMIT 1
 #define XG_OV_WRAP 2
 
-/* tile of one workgroup (256 work-items laid out 64 x 4) */
-#define XG_TX 64
 #define XG_THREADS 256
 
 namespace xg {
 
+/* Workgroup tile: 256 work-items, NC cells each.
+ *   1-D: 1024 consecutive cells (work-item t: t, t+256, t+512, t+768)
+ *   2-D: 4 rows x 64 columns (one cell per work-item)
+ *   3-D: 4 planes x 4 rows x 64 columns (work-item (r, c): planes 0..3) */
 template <int D> struct Tile;
-template <> struct Tile<1> { static constexpr int X = 1024, Y = 1, Z = 1; };
-template <> struct Tile<2> { static constexpr int X = 64, Y = 16, Z = 1; };
-template <> struct Tile<3> { static constexpr int X = 64, Y = 4, Z = 4; };
+template <> struct Tile<1> { static constexpr int E0 = 1024, E1 = 1, E2 = 1, NC = 4; };
+template <> struct Tile<2> { static constexpr int E0 = 4, E1 = 64, E2 = 1, NC = 1; };
+template <> struct Tile<3> { static constexpr int E0 = 4, E1 = 4, E2 = 64, NC = 4; };
 
 /* The hardware hands workgroup b to XCD (b mod 8).  Give each XCD one
  * contiguous run of tiles so vertical neighbours stay in that XCD's L2. */
@@ -45,10 +58,10 @@ __device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t nb) {
 }
 
 template <int D>
-__device__ __forceinline__ int64_t cells(const xg_gview& g) {
-  int64_t n = g.shape[0];
-  if (D > 1) n *= g.shape[1];
-  if (D > 2) n *= g.shape[2];
+__device__ __forceinline__ int64_t cells(const int32_t* shape) {
+  int64_t n = shape[0];
+  if (D > 1) n *= shape[1];
+  if (D > 2) n *= shape[2];
   return n;
 }
 
@@ -73,12 +86,30 @@ __device__ __forceinline__ int64_t index(const xg_gview& g, int i0, int i1, int 
   return lin<D>(g.shape, i0, i1, i2);
 }
 
-/* Load one cell.  Addresses outside the array (undefined in the reference's
- * C) are clamped to the array so the device never reads out of bounds. */
-template <class T, int D, int OV>
-__device__ __forceinline__ T ld(const xg_gview& g, int i0, int i1, int i2) {
+/* Linear offset of a stencil offset under the strides of `shape`. */
+template <int D>
+__device__ __forceinline__ int64_t off(const int32_t* shape, int o0, int o1, int o2) {
+  if (D == 1) return o0;
+  if (D == 2) return (int64_t)o0 * shape[1] + o1;
+  return ((int64_t)o0 * shape[1] + o1) * shape[2] + o2;
+}
+
+/* Load cell (i0 + o0, i1 + o1, i2 + o2).
+ * SAFE: the caller proved the cell lies inside the grid and that every view
+ * has the store grid's shape `sh`; `xc` is the linear index of (i0, i1, i2), so
+ * the address is xc plus a constant offset times the (scalar) strides.
+ * Otherwise the overstep rule applies, and addresses still outside the array
+ * (undefined in the reference's C) are clamped so the device never reads out
+ * of bounds. */
+template <class T, int D, int OV, bool SAFE>
+__device__ __forceinline__ T ld(const xg_gview& g, const int32_t* sh, int64_t xc, int o0, int o1, int o2,
+                                int i0, int i1, int i2) {
+  if (SAFE) return static_cast<const T*>(g.ptr)[xc + off<D>(sh, o0, o1, o2)];
+  i0 += o0;
+  i1 += o1;
+  i2 += o2;
   int64_t k = index<D, OV>(g, i0, i1, i2);
-  const int64_t n = cells<D>(g);
+  const int64_t n = cells<D>(g.shape);
   k = k < 0 ? 0 : (k >= n ? n - 1 : k);
   return static_cast<const T*>(g.ptr)[k];
 }
@@ -87,19 +118,23 @@ __device__ __forceinline__ T ld(const xg_gview& g, int i0, int i1, int i2) {
 template <class T, int D, int OV>
 __device__ __forceinline__ void st(const xg_gview& g, int i0, int i1, int i2, const T& v) {
   const int64_t k = index<D, OV>(g, i0, i1, i2);
-  if (k >= 0 && k < cells<D>(g)) static_cast<T*>(g.ptr)[k] = v;
+  if (k >= 0 && k < cells<D>(g.shape)) static_cast<T*>(g.ptr)[k] = v;
 }
 
 template <class S, int OV>
-__device__ __forceinline__ void apply(const xg_launch_hdr& H, const typename S::Refs& R,
-                                      const typename S::Params& P, int i0, int i1, int i2) {
+__device__ __forceinline__ void store(const xg_launch_hdr& H, int i0, int i1, int i2, const typename S::T& v) {
   typedef typename S::T T;
-  const T v = S::value(R, P, i0, i1, i2);
   if (H.dom.flags & XG_F_TO_TMP) {
     static_cast<T*>(H.src.ptr)[lin<S::D>(H.out.shape, i0, i1, i2)] = v;
   } else {
     st<T, S::D, OV>(H.out, i0 + S::SO0, i1 + S::SO1, i2 + S::SO2, v);
   }
+}
+
+template <class S, int OV>
+__device__ __forceinline__ void apply(const xg_launch_hdr& H, const typename S::Refs& R,
+                                      const typename S::Params& P, int i0, int i1, int i2) {
+  store<S, OV>(H, i0, i1, i2, S::template value<false>(R, P, H.out.shape, i0, i1, i2));
 }
 
 template <int D>
@@ -115,47 +150,95 @@ __device__ __forceinline__ void unlin(const int32_t* shape, int64_t c, int& i0, 
   }
 }
 
-/* Visit every cell of the box [lo, hi) handled by this workgroup's tile. */
-template <int D, class F>
-__device__ __forceinline__ void for_tile(const int32_t* lo, const int32_t* hi, F&& f) {
-  typedef Tile<D> TL;
-  const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
-  const int tid = threadIdx.x;
-  if (D == 1) {
-    const int base = lo[0] + (int)tile * TL::X + tid;
-#pragma unroll
-    for (int r = 0; r < TL::X / XG_THREADS; ++r) {
-      const int i0 = base + r * XG_THREADS;
-      if (i0 < hi[0]) f(i0, 0, 0);
+/* This workgroup's tile of the box [lo, hi): the NC cells of this work-item,
+ * whether each is inside the box, and the tile's extent clipped to the box. */
+template <int D>
+struct TileCells {
+  static constexpr int NC = Tile<D>::NC;
+  int c[NC][3];
+  bool in[NC];
+  int32_t tlo[3], thi[3];
+
+  __device__ __forceinline__ TileCells(const int32_t* lo, const int32_t* hi) {
+    typedef Tile<D> TL;
+    const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
+    const int tid = threadIdx.x;
+    int t0 = (int)tile, t1 = 0, t2 = 0;
+    if (D == 2) {
+      const int n1 = (hi[1] - lo[1] + TL::E1 - 1) / TL::E1;
+      t0 = (int)tile / n1;
+      t1 = (int)tile - t0 * n1;
+    } else if (D == 3) {
+      const int n2 = (hi[2] - lo[2] + TL::E2 - 1) / TL::E2;
+      const int n1 = (hi[1] - lo[1] + TL::E1 - 1) / TL::E1;
+      t0 = (int)tile / (n1 * n2);
+      const int rem = (int)tile - t0 * n1 * n2;
+      t1 = rem / n2;
+      t2 = rem - t1 * n2;
     }
-  } else if (D == 2) {
-    const int nx = (hi[1] - lo[1] + TL::X - 1) / TL::X;
-    const int t0 = (int)tile / nx, t1 = (int)tile - t0 * nx;
-    const int i1 = lo[1] + t1 * TL::X + (tid & 63);
-    const int r0 = lo[0] + t0 * TL::Y + (tid >> 6) * (TL::Y / 4);
-    if (i1 < hi[1]) {
+    tlo[0] = lo[0] + t0 * TL::E0;
+    tlo[1] = D > 1 ? lo[1] + t1 * TL::E1 : 0;
+    tlo[2] = D > 2 ? lo[2] + t2 * TL::E2 : 0;
+    thi[0] = min(tlo[0] + TL::E0, hi[0]);
+    thi[1] = D > 1 ? min(tlo[1] + TL::E1, hi[1]) : 1;
+    thi[2] = D > 2 ? min(tlo[2] + TL::E2, hi[2]) : 1;
 #pragma unroll
-      for (int r = 0; r < TL::Y / 4; ++r) {
-        const int i0 = r0 + r;
-        if (i0 < hi[0]) f(i0, i1, 0);
+    for (int n = 0; n < NC; ++n) {
+      if (D == 1) {
+        c[n][0] = tlo[0] + tid + n * XG_THREADS;
+        c[n][1] = 0;
+        c[n][2] = 0;
+      } else if (D == 2) {
+        c[n][0] = tlo[0] + (tid >> 6) + 4 * n;  /* NC = 1: one row per wave */
+        c[n][1] = tlo[1] + (tid & 63);
+        c[n][2] = 0;
+      } else {
+        c[n][0] = tlo[0] + n;
+        c[n][1] = tlo[1] + (tid >> 6);
+        c[n][2] = tlo[2] + (tid & 63);
       }
-    }
-  } else {
-    const int nx = (hi[2] - lo[2] + TL::X - 1) / TL::X;
-    const int ny = (hi[1] - lo[1] + TL::Y - 1) / TL::Y;
-    const int t0 = (int)tile / (nx * ny);
-    const int rem = (int)tile - t0 * nx * ny;
-    const int t1 = rem / nx, t2 = rem - t1 * nx;
-    const int i2 = lo[2] + t2 * TL::X + (tid & 63);
-    const int i1 = lo[1] + t1 * TL::Y + (tid >> 6);
-    if (i2 < hi[2] && i1 < hi[1]) {
-#pragma unroll
-      for (int r = 0; r < TL::Z; ++r) {
-        const int i0 = lo[0] + t0 * TL::Z + r;
-        if (i0 < hi[0]) f(i0, i1, i2);
-      }
+      in[n] = c[n][0] < thi[0] && (D < 2 || c[n][1] < thi[1]) && (D < 3 || c[n][2] < thi[2]);
     }
   }
+};
+
+/* Do the tile's cells, the statement's loads and its store all lie inside the
+ * grid, with every view of the grid shape (so no clamp or wrap is needed)? */
+template <class S>
+__device__ __forceinline__ bool tile_safe(const xg_launch_hdr& H, const int32_t* tlo, const int32_t* thi) {
+  if (!(H.dom.flags & XG_F_UNIFORM)) return false;
+  const int32_t* sh = H.out.shape;
+  bool ok = tlo[0] - S::RL0 >= 0 && thi[0] + S::RH0 <= sh[0];
+  if (S::D > 1) ok = ok && tlo[1] - S::RL1 >= 0 && thi[1] + S::RH1 <= sh[1];
+  if (S::D > 2) ok = ok && tlo[2] - S::RL2 >= 0 && thi[2] + S::RH2 <= sh[2];
+  return ok;
+}
+
+template <class S, int OV, bool MASKED>
+__device__ __forceinline__ void box_cells(const xg_launch_hdr& H, const typename S::Refs& R,
+                                          const typename S::Params& P) {
+  constexpr int D = S::D;
+  typedef typename S::T T;
+  TileCells<D> tc(H.dom.lo, H.dom.hi);
+  constexpr int NC = TileCells<D>::NC;
+  if (MASKED) {
+#pragma unroll
+    for (int n = 0; n < NC; ++n)
+      tc.in[n] = tc.in[n] && H.dom.mask[lin<D>(H.out.shape, tc.c[n][0], tc.c[n][1], tc.c[n][2])] == H.dom.klass;
+  }
+  T v[NC];
+  if (tile_safe<S>(H, tc.tlo, tc.thi)) {
+#pragma unroll
+    for (int n = 0; n < NC; ++n)
+      if (tc.in[n]) v[n] = S::template value<true>(R, P, H.out.shape, tc.c[n][0], tc.c[n][1], tc.c[n][2]);
+  } else {
+#pragma unroll
+    for (int n = 0; n < NC; ++n)
+      if (tc.in[n]) v[n] = S::template value<false>(R, P, H.out.shape, tc.c[n][0], tc.c[n][1], tc.c[n][2]);
+  }
+#pragma unroll
+  for (int n = 0; n < NC; ++n)
+    if (tc.in[n]) store<S, OV>(H, tc.c[n][0], tc.c[n][1], tc.c[n][2], v[n]);
 }
 
 template <class S, int OV>
@@ -170,14 +253,9 @@ __device__ void run_main(const xg_launch_hdr& H, const typename S::Refs& R, cons
       apply<S, OV>(H, R, P, i0, i1, i2);
     }
   } else if (dom.kind == XG_DOM_MASKED) {
-    const int32_t* mask = dom.mask;
-    const int klass = dom.klass;
-    const int32_t* shape = H.out.shape;
-    for_tile<D>(dom.lo, dom.hi, [&](int i0, int i1, int i2) {
-      if (mask[lin<D>(shape, i0, i1, i2)] == klass) apply<S, OV>(H, R, P, i0, i1, i2);
-    });
+    box_cells<S, OV, true>(H, R, P);
   } else {
-    for_tile<D>(dom.lo, dom.hi, [&](int i0, int i1, int i2) { apply<S, OV>(H, R, P, i0, i1, i2); });
+    box_cells<S, OV, false>(H, R, P);
   }
 }
 
@@ -192,18 +270,45 @@ __device__ void run_pingpong(const xg_launch_hdr& H, const typename S::Refs& R, 
   const int32_t* shape = H.out.shape;
   const T* __restrict__ src = static_cast<const T*>(H.src.ptr);
   T* __restrict__ out = static_cast<T*>(H.out.ptr);
-  for_tile<D>(zero, shape, [&](int i0, int i1, int i2) {
-    const int64_t c = lin<D>(shape, i0, i1, i2);
-    bool member;
+  TileCells<D> tc(zero, shape);
+  constexpr int NC = TileCells<D>::NC;
+  bool member[NC];
+  int64_t lc[NC];
+#pragma unroll
+  for (int n = 0; n < NC; ++n) {
+    const int* c = tc.c[n];
+    lc[n] = lin<D>(shape, c[0], c[1], c[2]);
     if (dom.kind == XG_DOM_BOX) {
-      member = i0 >= dom.lo[0] && i0 < dom.hi[0];
-      if (D > 1) member = member && i1 >= dom.lo[1] && i1 < dom.hi[1];
-      if (D > 2) member = member && i2 >= dom.lo[2] && i2 < dom.hi[2];
+      bool m = c[0] >= dom.lo[0] && c[0] < dom.hi[0];
+      if (D > 1) m = m && c[1] >= dom.lo[1] && c[1] < dom.hi[1];
+      if (D > 2) m = m && c[2] >= dom.lo[2] && c[2] < dom.hi[2];
+      member[n] = tc.in[n] && m;
     } else {
-      member = dom.mask[c] == dom.klass;
+      member[n] = tc.in[n] && dom.mask[lc[n]] == dom.klass;
     }
-    out[c] = member ? S::value(R, P, i0, i1, i2) : src[c];
-  });
+  }
+  T v[NC];
+  if (tile_safe<S>(H, tc.tlo, tc.thi)) {
+#pragma unroll
+    for (int n = 0; n < NC; ++n)
+      if (tc.in[n]) v[n] = member[n] ? S::template value<true>(R, P, H.out.shape, tc.c[n][0], tc.c[n][1], tc.c[n][2]) : src[lc[n]];
+  } else {
+#pragma unroll
+    for (int n = 0; n < NC; ++n)
+      if (tc.in[n]) v[n] = member[n] ? S::template value<false>(R, P, H.out.shape, tc.c[n][0], tc.c[n][1], tc.c[n][2]) : src[lc[n]];
+  }
+#pragma unroll
+  for (int n = 0; n < NC; ++n)
+    if (tc.in[n]) out[lc[n]] = v[n];
+}
+
+/* Visit every cell of the box [lo, hi) handled by this workgroup's tile. */
+template <int D, class F>
+__device__ __forceinline__ void for_tile(const int32_t* lo, const int32_t* hi, F&& f) {
+  TileCells<D> tc(lo, hi);
+#pragma unroll
+  for (int n = 0; n < TileCells<D>::NC; ++n)
+    if (tc.in[n]) f(tc.c[n][0], tc.c[n][1], tc.c[n][2]);
 }
 
 /* Phase 2 of a two-phase statement: store the temporary at the shifted cell. */

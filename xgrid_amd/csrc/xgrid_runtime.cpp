@@ -180,6 +180,7 @@ struct xg_kernel {
   int32_t timed_sid = -1;
   std::vector<EventPair> events;
   std::vector<int64_t> exec_count;  // stmt() calls per statement (all modes)
+  std::vector<std::vector<uint8_t>> last_params;  // parameters of each statement's last call
   std::vector<uint8_t> key_buf;
 };
 
@@ -280,6 +281,19 @@ void fill_view(xg_gview& v, const xg_grid* G, void* ptr) {
   for (int d = 0; d < 3; ++d) v.shape[d] = d < G->dims ? G->shape[d] : 1;
 }
 
+// Does every array the statement loads have the store grid's rank and shape?
+bool uniform_refs(xg_kernel* k, const xg_stmt_desc& d) {
+  const xg_grid* G = k->grids[(size_t)d.store_arg];
+  for (int r = 0; r < d.nrefs; ++r) {
+    if (d.ref_level[r] < 0) continue;
+    const xg_grid* R = k->grids[(size_t)d.ref_arg[r]];
+    if (R->dims != G->dims) return false;
+    for (int a = 0; a < G->dims; ++a)
+      if (R->shape[a] != G->shape[a]) return false;
+  }
+  return true;
+}
+
 int fill_refs(xg_kernel* k, const xg_stmt_desc& d, std::vector<xg_gview>& refs) {
   refs.assign((size_t)std::max(1, d.nrefs), xg_gview{});
   for (int r = 0; r < d.nrefs; ++r) {
@@ -298,7 +312,7 @@ int fill_refs(xg_kernel* k, const xg_stmt_desc& d, std::vector<xg_gview>& refs) 
 // workgroups covering the box [lo, hi) with the tile of a D-dimensional statement
 int64_t tiles_for(int D, const int32_t* lo, const int32_t* hi) {
   if (D == 1) return ceil_div(hi[0] - lo[0], 1024);
-  if (D == 2) return ceil_div(hi[0] - lo[0], 16) * ceil_div(hi[1] - lo[1], 64);
+  if (D == 2) return ceil_div(hi[0] - lo[0], 4) * ceil_div(hi[1] - lo[1], 64);
   return ceil_div(hi[0] - lo[0], 4) * ceil_div(hi[1] - lo[1], 4) * ceil_div(hi[2] - lo[2], 64);
 }
 
@@ -341,6 +355,8 @@ int exec_stmt(xg_kernel* k, int32_t sid, const void* params) {
   int rc = fill_refs(k, d, refs);
   if (rc) return rc;
   void* args[3] = {&H, refs.data(), const_cast<void*>(params)};
+  const int32_t uni = uniform_refs(k, d) ? XG_F_UNIFORM : 0;
+  H.dom.flags = uni;
   if (mode == XG_MODE_INPLACE) {
     fill_view(H.out, G, G->levels[0]);
     return do_launch(k, sid, d.fn_main, blocks_for_dom(d.dims, H.dom), kThreads, args);
@@ -349,7 +365,7 @@ int exec_stmt(xg_kernel* k, int32_t sid, const void* params) {
   if (mode == XG_MODE_PINGPONG) {
     if (!d.fn_pingpong) return fail(XG_ERR_STATE, "statement has no Jacobi kernel");
     if (H.dom.kind == XG_DOM_LIST) H.dom.kind = XG_DOM_MASKED;  // membership by mask
-    H.dom.flags = XG_F_PINGPONG;
+    H.dom.flags = XG_F_PINGPONG | uni;
     fill_view(H.out, G, G->scratch);
     fill_view(H.src, G, G->levels[0]);
     int32_t zero[3] = {0, 0, 0};
@@ -361,7 +377,7 @@ int exec_stmt(xg_kernel* k, int32_t sid, const void* params) {
   }
   // two-phase: value -> scratch[cell], then scratch[cell] -> store(cell + offset)
   if (!d.fn_phase2) return fail(XG_ERR_STATE, "statement has no phase-2 kernel");
-  H.dom.flags = XG_F_TO_TMP;
+  H.dom.flags = XG_F_TO_TMP | uni;
   fill_view(H.out, G, G->levels[0]);
   fill_view(H.src, G, G->scratch);
   rc = do_launch(k, sid, d.fn_main, blocks_for_dom(d.dims, H.dom), kThreads, args);
@@ -553,6 +569,7 @@ int api_begin(xg_kernel* k, xg_grid* const* grids, int32_t ngrids, const void* k
 int api_stmt(xg_kernel* k, int32_t sid, const void* params, int64_t bytes) {
   if (k->status != XG_OK) return k->status;
   if (sid < 0 || sid >= (int32_t)k->stmts.size()) return set_status(k, fail(XG_ERR_ARG, "bad statement id"));
+  k->last_params[(size_t)sid].assign((const uint8_t*)params, (const uint8_t*)params + bytes);
   if (k->mode == kReplay) {
     k->exec_count[(size_t)sid]++;
     return XG_OK;
@@ -951,6 +968,7 @@ int xg_kernel_create(int32_t nstmts, const xg_stmt_desc* stmts, int32_t ngroups,
     k->groups.push_back(groups[i]);
   }
   k->exec_count.assign((size_t)nstmts, 0);
+  k->last_params.assign((size_t)nstmts, std::vector<uint8_t>());
   *out = k;
   return XG_OK;
 }
@@ -1021,6 +1039,34 @@ int xg_kernel_stmt_counts(xg_kernel* k, int64_t* counts, int32_t n, int32_t rese
   for (int32_t i = 0; i < n && i < (int32_t)k->exec_count.size(); ++i) counts[i] = k->exec_count[(size_t)i];
   if (reset) std::fill(k->exec_count.begin(), k->exec_count.end(), 0);
   return XG_OK;
+}
+
+int xg_kernel_time_isolated(xg_kernel* k, int32_t sid, int32_t reps, double* ms_per_launch) {
+  if (int rc = check_ready()) return rc;
+  if (!k || sid < 0 || sid >= (int32_t)k->stmts.size() || reps < 1) return fail(XG_ERR_ARG, "invalid arguments");
+  if (k->grids.empty() || k->last_params[(size_t)sid].empty())
+    return fail(XG_ERR_STATE, "the statement has not run yet");
+  if (k->mode != kDirect) return fail(XG_ERR_STATE, "a call is in progress");
+  const std::vector<uint8_t> params = k->last_params[(size_t)sid];
+  const int32_t saved = k->timed_sid;
+  k->timed_sid = -1;
+  hipEvent_t e0, e1;
+  XG_HIP(hipEventCreate(&e0));
+  XG_HIP(hipEventCreate(&e1));
+  int rc = exec_stmt(k, sid, params.data());  // warm
+  if (!rc) {
+    XG_HIP(hipEventRecord(e0, g_dev.stream));
+    for (int i = 0; i < reps && !rc; ++i) rc = exec_stmt(k, sid, params.data());
+    XG_HIP(hipEventRecord(e1, g_dev.stream));
+    XG_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    XG_HIP(hipEventElapsedTime(&ms, e0, e1));
+    if (ms_per_launch) *ms_per_launch = (double)ms / reps;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  k->timed_sid = saved;
+  return rc;
 }
 
 const xg_api* xg_api_table(void) { return &g_api; }

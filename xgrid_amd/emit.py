@@ -252,9 +252,10 @@ class Emitter:
             gt = grids[e.grid.name]
             if gt.dimension > D:
                 raise EmitError(f"{e.loc}: grid '{e.grid.name}' has more dimensions than the store grid")
-            idx = [f"i{d} + {e.offsets[d]}" if d < gt.dimension else "0" for d in range(3)]
+            offs = [str(e.offsets[d]) if d < gt.dimension else "0" for d in range(3)]
             et = device_type_name(gt.element)
-            return f"xg::ld<{et}, {gt.dimension}, XG_OVERSTEP>(R.v[{r}], {', '.join(idx)})"
+            return (f"xg::ld<{et}, {gt.dimension}, XG_OVERSTEP, SAFE>(R.v[{r}], sh, xc, {', '.join(offs)}, "
+                    f"i0, i1, i2)")
 
         def name(e: ir.Name) -> str:
             return f"P.{e.var.name}"
@@ -265,6 +266,12 @@ class Emitter:
 
         expr = _ExprC(True, cell, name, grid_info)(st.node.value)
         so = list(st.store_offsets) + [0] * (3 - len(st.store_offsets))
+        rl, rh = [0, 0, 0], [0, 0, 0]
+        for n in ir.walk(st.node.value):
+            if isinstance(n, ir.Cell):
+                for d, o in enumerate(n.offsets):
+                    rl[d] = max(rl[d], -o)
+                    rh[d] = max(rh[d], o)
         nr = max(1, len(refs))
         out.append(f"// statement {sid}: {st.node.loc}")
         if params:
@@ -276,8 +283,13 @@ class Emitter:
         out.append(f"struct xg_s{sid} {{")
         out.append(f"  typedef {T} T; typedef xg_r{sid} Refs; typedef xg_p{sid} Params;")
         out.append(f"  static constexpr int D = {D}, SO0 = {so[0]}, SO1 = {so[1]}, SO2 = {so[2]};")
-        out.append(f"  __device__ static __forceinline__ T value(const Refs& R, const Params& P, int i0, int i1, int i2) {{")
-        out.append("    (void)R; (void)P; (void)i0; (void)i1; (void)i2;")
+        out.append(f"  static constexpr int RL0 = {rl[0]}, RL1 = {rl[1]}, RL2 = {rl[2]}, "
+                   f"RH0 = {rh[0]}, RH1 = {rh[1]}, RH2 = {rh[2]};")
+        out.append("  template <bool SAFE>")
+        out.append(f"  __device__ static __forceinline__ T value(const Refs& R, const Params& P, const int32_t* sh, "
+                   f"int i0, int i1, int i2) {{")
+        out.append(f"    const int64_t xc = SAFE ? xg::lin<{D}>(sh, i0, i1, i2) : 0;")
+        out.append("    (void)R; (void)P; (void)sh; (void)xc; (void)i0; (void)i1; (void)i2;")
         out.append(f"    return {expr};")
         out.append("  }")
         out.append("};")
