@@ -66,13 +66,31 @@ typedef struct xg_group_desc {
   void* fn;                                  /* hipFunction_t taking (uint32 enable, {hdr, refs, params} x count) */
 } xg_group_desc;
 
+/* A `for` loop whose body is stencil statements on one 2-D grid, which may run
+ * temporally blocked: `tmax` iterations per launch on core0 x core1 tiles with
+ * a halo of `halo` cells kept in LDS (reference: the same statements as
+ * separate nests, xgrid/lang/generator.py:369-374 around :289-361). */
+#define XG_LOOP_MAX_STMTS 8
+typedef struct xg_loop_desc {
+  int32_t nstmts;
+  int32_t sids[XG_LOOP_MAX_STMTS];
+  int32_t grid_arg;
+  int32_t core0, core1, halo, tmax;
+  int32_t threads;                           /* work-items per workgroup */
+  void* fn;                                  /* hipFunction_t: tmax-bounded blocked iterations */
+  void* fn_check;                            /* the same, proving every written core cell exact */
+} xg_loop_desc;
+
 /* Function table handed to generated host stubs (the stub runs the kernel's
- * scalar control flow and issues one stmt() per executed stencil statement). */
+ * scalar control flow and issues one stmt() per executed stencil statement;
+ * for a blockable loop it first offers the whole loop to loop(), which returns
+ * 1 when it ran all `trip` iterations and 0 when the stub must run them). */
 typedef struct xg_api {
   int (*begin)(xg_kernel* k, xg_grid* const* grids, int32_t ngrids, const void* key, int64_t key_bytes);
   int (*stmt)(xg_kernel* k, int32_t sid, const void* params, int64_t param_bytes);
   int (*end)(xg_kernel* k);
   const int32_t* (*shape)(xg_grid* g);
+  int (*loop)(xg_kernel* k, int32_t lid, int64_t trip, const void* const* params, int32_t nparams);
 } xg_api;
 
 /* ---- errors and device ------------------------------------------------- */
@@ -121,16 +139,26 @@ void* xg_grid_level_ptr(xg_grid* g, int32_t level);
 /* Create the launch plan of a kernel from its statements.  Replaces the
  * native handle Generator(op).result returns (xgrid/lang/operator.py:31-34). */
 int xg_kernel_create(int32_t nstmts, const xg_stmt_desc* stmts, int32_t ngroups,
-                     const xg_group_desc* groups, int32_t ngrids, xg_kernel** out);
+                     const xg_group_desc* groups, int32_t nloops, const xg_loop_desc* loops,
+                     int32_t ngrids, xg_kernel** out);
 int xg_kernel_destroy(xg_kernel* k);
 /* Status of the last call made through the kernel's stub (0 or XG_ERR_*). */
 int xg_kernel_status(xg_kernel* k);
 /* Options: graphs (replay repeated calls as hipGraphs), fuse (chain small
- * boundary statements in one workgroup). */
-int xg_kernel_set_options(xg_kernel* k, int32_t graphs, int32_t fuse);
+ * boundary statements in one workgroup), blocked (run blockable loops
+ * temporally blocked). */
+int xg_kernel_set_options(xg_kernel* k, int32_t graphs, int32_t fuse, int32_t blocked);
 /* Launch counters and graph cache statistics. */
 int xg_kernel_counters(xg_kernel* k, int64_t* launches, int64_t* fused, int64_t* graph_replays,
                        int64_t* graph_captures);
+/* Blocked-loop statistics: launches, iterations run blocked, and the plan
+ * state of loop `lid` (1 exact and in use, 0 not yet validated, -1 declined). */
+int xg_kernel_loop_info(xg_kernel* k, int32_t lid, int64_t* launches, int64_t* iterations, int32_t* state);
+/* Run loop `lid` blocked for `trip` iterations (the launches a call makes:
+ * chunks of tmax iterations), `reps` times back to back, on the grids and
+ * parameters of its last call, between two HIP events; returns ms per launch.
+ * Advances the grid's state. */
+int xg_kernel_time_loop(xg_kernel* k, int32_t lid, int32_t trip, int32_t reps, double* ms_per_launch);
 /* Bracket every direct launch of statement `sid` with HIP events on the
  * backend stream; xg_kernel_stmt_time returns the summed kernel time. */
 int xg_kernel_time_stmt(xg_kernel* k, int32_t sid, int32_t enable);

@@ -275,7 +275,10 @@ def test_graphs_and_fusion_do_not_change_results(double):
         gs = [xgrid.Grid((n, n), float) for _ in "bpuv"]
         for g, c in zip(gs, "bpuv"):
             g.boundary[...] = masks[c]
-        for _ in range(6):
+        # the buffer states of a call repeat with a period of up to 6 calls (ticks of
+        # 2-level grids, 3-buffer rotations of the Jacobi grid); the third sighting
+        # of a state replays its graph
+        for _ in range(14):
             k(*gs, cfg)
         out.append([g.now.copy() for g in gs])
         if graphs:
@@ -317,3 +320,54 @@ def test_control_flow_and_shape_intrinsics(double):
     d, h = grid_pair(xgrid, (5, 12), float, None, rng.random((5, 12)))
     run_pair(k, (d, 5), (h, 5), steps=2)
     assert_bitexact(d, h)
+
+
+def test_blocked_loop_matches_unblocked_and_oracle(double):
+    from xgrid_amd import workloads as W
+    n, steps = 97, 4
+    cfg = W.cavity_config(n, dt=2e-6)
+    masks = W.cavity_masks(n)
+    res = []
+    for blocked in (True, False):
+        xgrid.init(precision="double", blocked=blocked)
+        k = W.make_cavity()
+        pairs = {c: grid_pair(xgrid, (n, n), float, masks[c]) for c in "bpuv"}
+        run_pair(k, [pairs[c][0] for c in "bpuv"] + [cfg], [pairs[c][1] for c in "bpuv"] + [cfg], steps=steps)
+        for c in "bpuv":
+            assert_bitexact(*pairs[c], levels=[0])
+        info = k.native.loop_info(0)
+        if blocked:
+            assert info["state"] == 1 and info["iterations"] == 50 * steps
+        else:
+            assert info["iterations"] == 0
+        res.append([pairs[c][0].now.copy() for c in "bpuv"])
+    for a, b in zip(*res):
+        assert np.array_equal(a, b)
+
+
+def test_blocked_loop_declined_when_halo_too_small(double):
+    """Three hops along a boundary row per iteration (classes 1, 2, 3 cycling
+    along row 0, each statement reading the class before it) outrun the halo
+    (one cell per iteration): validation must decline the blocked plan and the
+    loop falls back to per-statement launches with identical results."""
+    @xgrid.kernel()
+    def creep(u: f2, v: f2) -> None:
+        for _ in range(0, 12):
+            u[0, 0] = (u[0, 1][0] + u[0, -1][0] + u[1, 0][0] + u[-1, 0][0]) * 0.25 + v[0, 0]
+            with xgrid.boundary(1):
+                u[0, 0] = u[0, -1][0] + 1.0
+            with xgrid.boundary(2):
+                u[0, 0] = u[0, -1][0] * 0.5
+            with xgrid.boundary(3):
+                u[0, 0] = u[0, -1][0] - 0.25
+
+    n = 150
+    mask = np.zeros((n, n), np.int32)
+    mask[0, :] = 1 + np.arange(n) % 3
+    mask[-1, :] = mask[:, 0] = mask[:, -1] = 4
+    rng = np.random.default_rng(12)
+    du, hu = grid_pair(xgrid, (n, n), float, mask, rng.random((n, n)))
+    dv, hv = grid_pair(xgrid, (n, n), float, None, rng.random((n, n)) * 1e-3)
+    run_pair(creep, (du, dv), (hu, hv), steps=2)
+    assert creep.native.loop_info(0)["state"] == -1
+    assert_bitexact(du, hu)

@@ -39,8 +39,18 @@ class GroupDesc(Structure):
     _fields_ = [("first", c_int32), ("count", c_int32), ("fn", c_void_p)]
 
 
+XG_LOOP_MAX = 8
+
+
+class LoopDesc(Structure):
+    _fields_ = [("nstmts", c_int32), ("sids", c_int32 * XG_LOOP_MAX), ("grid_arg", c_int32),
+                ("core0", c_int32), ("core1", c_int32), ("halo", c_int32), ("tmax", c_int32),
+                ("threads", c_int32), ("fn", c_void_p), ("fn_check", c_void_p)]
+
+
 class Api(Structure):
-    _fields_ = [("begin", c_void_p), ("stmt", c_void_p), ("end", c_void_p), ("shape", c_void_p)]
+    _fields_ = [("begin", c_void_p), ("stmt", c_void_p), ("end", c_void_p), ("shape", c_void_p),
+                ("loop", c_void_p)]
 
 
 _SIGS = {
@@ -66,11 +76,13 @@ _SIGS = {
     "xg_grid_class_info": ([c_void_p, c_int32, POINTER(c_int32), POINTER(c_int64), POINTER(c_int32),
                             POINTER(c_int32)], c_int),
     "xg_grid_level_ptr": ([c_void_p, c_int32], c_void_p),
-    "xg_kernel_create": ([c_int32, POINTER(StmtDesc), c_int32, POINTER(GroupDesc), c_int32,
-                          POINTER(c_void_p)], c_int),
+    "xg_kernel_create": ([c_int32, POINTER(StmtDesc), c_int32, POINTER(GroupDesc), c_int32, POINTER(LoopDesc),
+                          c_int32, POINTER(c_void_p)], c_int),
     "xg_kernel_destroy": ([c_void_p], c_int),
     "xg_kernel_status": ([c_void_p], c_int),
-    "xg_kernel_set_options": ([c_void_p, c_int32, c_int32], c_int),
+    "xg_kernel_set_options": ([c_void_p, c_int32, c_int32, c_int32], c_int),
+    "xg_kernel_loop_info": ([c_void_p, c_int32, POINTER(c_int64), POINTER(c_int64), POINTER(c_int32)], c_int),
+    "xg_kernel_time_loop": ([c_void_p, c_int32, c_int32, c_int32, POINTER(c_double)], c_int),
     "xg_kernel_counters": ([c_void_p, POINTER(c_int64), POINTER(c_int64), POINTER(c_int64),
                             POINTER(c_int64)], c_int),
     "xg_kernel_time_stmt": ([c_void_p, c_int32, c_int32], c_int),

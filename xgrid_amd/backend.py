@@ -110,11 +110,24 @@ class HipKernel:
             groups[i].first = g.first
             groups[i].count = g.count
             groups[i].fn = fn(g.kernel)
+        loops = (_lib.LoopDesc * max(1, len(low.loops)))()
+        for i, L in enumerate(low.loops):
+            ld = loops[i]
+            ld.nstmts = len(L.stmts)
+            for j, m in enumerate(L.stmts):
+                ld.sids[j] = m.stmt.sid
+            ld.grid_arg = grid_index[L.grid]
+            ld.core0, ld.core1 = L.core
+            ld.halo = L.halo
+            ld.tmax = L.tmax
+            ld.threads = L.threads
+            ld.fn = fn(L.kernel)
+            ld.fn_check = fn(L.kernel + "_chk")
         k = c_void_p()
-        _lib.check(self.lib.xg_kernel_create(len(low.stmts), descs, len(low.groups), groups, len(low.grid_args),
-                                             ctypes.byref(k)), "xg_kernel_create")
+        _lib.check(self.lib.xg_kernel_create(len(low.stmts), descs, len(low.groups), groups, len(low.loops), loops,
+                                             len(low.grid_args), ctypes.byref(k)), "xg_kernel_create")
         self.kernel = k.value
-        _lib.check(self.lib.xg_kernel_set_options(self.kernel, int(cfg.graphs and low.graphs_ok), int(cfg.fuse)))
+        self.set_options()
         self.api = self.lib.xg_api_table()
         self.stub = load_host(self.stub_path)
         self.entry = self.stub.xg_entry
@@ -131,11 +144,24 @@ class HipKernel:
             fields_ = [("xg_unused", ctypes.c_int32)]
         return ctypes.sizeof(type("P", (ctypes.Structure,), {"_fields_": fields_}))
 
-    def set_options(self, graphs: bool | None = None, fuse: bool | None = None):
+    def set_options(self, graphs: bool | None = None, fuse: bool | None = None, blocked: bool | None = None):
         cfg = get_config()
         g = cfg.graphs and self.lowered.graphs_ok if graphs is None else graphs
         f = cfg.fuse if fuse is None else fuse
-        _lib.check(self.lib.xg_kernel_set_options(self.kernel, int(g), int(f)))
+        b = cfg.blocked if blocked is None else blocked
+        _lib.check(self.lib.xg_kernel_set_options(self.kernel, int(g), int(f), int(b)))
+
+    def loop_info(self, lid: int = 0) -> dict:
+        n, it, st = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int32()
+        _lib.check(self.lib.xg_kernel_loop_info(self.kernel, lid, ctypes.byref(n), ctypes.byref(it), ctypes.byref(st)))
+        return {"launches": n.value, "iterations": it.value, "state": st.value}
+
+    def time_loop(self, lid: int, trip: int, reps: int = 50) -> float:
+        """ms per launch of loop `lid`'s blocked kernel, running `trip` iterations the
+        way a call does (chunks of at most tmax), back to back."""
+        ms = ctypes.c_double()
+        _lib.check(self.lib.xg_kernel_time_loop(self.kernel, lid, trip, reps, ctypes.byref(ms)))
+        return ms.value
 
     def counters(self) -> dict:
         vals = [ctypes.c_int64() for _ in range(4)]

@@ -35,7 +35,8 @@ class Configuration:
     device: int = 0
     fp_contract: bool = False      # FMA contraction in device code
     graphs: bool = True            # replay repeated calls as hipGraphs
-    fuse: bool = True              # fuse small boundary chains / Jacobi tails
+    fuse: bool = True              # chain small boundary statements in one workgroup
+    blocked: bool = True           # run blockable stencil loops temporally blocked
 
     def __repr__(self) -> str:
         return repr(asdict(self))
@@ -63,7 +64,7 @@ def init(*, parallel: bool = True, cc: list | None = None, cacheroot: str = ".xg
          comment: bool = False, overstep: Overstep = "none",
          opt_level: int = 2, precision: Precision = "float",
          device: int | None = None, fp_contract: bool = False,
-         graphs: bool | None = None, fuse: bool | None = None) -> None:
+         graphs: bool | None = None, fuse: bool | None = None, blocked: bool | None = None) -> None:
     """Initialise the global configuration (``xgrid/util/init.py:48-66``).
 
     ``parallel=False`` keeps the reference's rule that implicit (Jacobi)
@@ -83,8 +84,10 @@ def init(*, parallel: bool = True, cc: list | None = None, cacheroot: str = ".xg
         graphs = os.environ.get("XGRID_GRAPHS", "1") != "0"
     if fuse is None:
         fuse = os.environ.get("XGRID_FUSE", "1") != "0"
+    if blocked is None:
+        blocked = os.environ.get("XGRID_BLOCKED", "1") != "0"
     _config = Configuration(parallel=parallel, cc=list(cc or ["gcc", "clang"]),
                             cacheroot=cacheroot, comment=comment, overstep=overstep,
                             opt_level=int(opt_level), precision=precision, device=device,
-                            fp_contract=fp_contract, graphs=graphs, fuse=fuse)
+                            fp_contract=fp_contract, graphs=graphs, fuse=fuse, blocked=blocked)
     logger.info("initialized with configuration: %s", _config)

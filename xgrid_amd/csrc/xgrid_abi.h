@@ -62,4 +62,17 @@ typedef struct xg_launch_hdr {
   xg_gview src;              /* pingpong source / two-phase temporary */
 } xg_launch_hdr;             /* 120 bytes */
 
+/* Temporally blocked loop: several iterations of a loop body of stencil
+ * statements on one grid, per workgroup tile, in LDS. */
+#define XG_TB_MAX_STMTS 8
+typedef struct xg_tb_hdr {
+  xg_gview src;                    /* the grid's level 0 before the launch */
+  xg_gview dst;                    /* receives every cell after `iters` iterations */
+  const int32_t* mask;             /* boundary mask (null: every cell is class 0) */
+  int32_t* fail;                   /* validation launch: set when a core cell is not exact */
+  int32_t klass[XG_TB_MAX_STMTS];  /* class of each statement */
+  int32_t iters;                   /* iterations in this launch (<= the compiled maximum) */
+  int32_t uniform;                 /* every loaded view has the grid's shape */
+} xg_tb_hdr;                       /* 24 + 24 + 8 + 8 + 32 + 8 = 104 bytes */
+
 #endif

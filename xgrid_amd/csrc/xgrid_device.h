@@ -365,6 +365,39 @@ __device__ void run_block(const xg_launch_hdr& H, const typename S::Refs& R, con
 template <class T>
 __device__ __forceinline__ T sq(T x) { return x * x; }
 
+/* A compile-time boolean carried as a value (selects a template instantiation). */
+template <bool B>
+struct bconst {
+  static constexpr bool value = B;
+};
+
+/* ---- temporally blocked loops (2-D) -------------------------------------
+ * A workgroup owns a core tile of the grid and keeps core + halo (an RH x RW
+ * region) of the loop's grid in LDS for several iterations of the loop body.
+ * Region cells outside the grid hold 0 and are never valid.  Only cells at
+ * least the body's reach inside the region are computed, so a read of the
+ * grid at region coordinates (r, c) is always inside the region.  With CHECK,
+ * `ok` records whether the value read is exact (valid), so a validation launch
+ * can prove that every core cell it writes is exact. */
+template <class T, bool CHECK, int RH, int RW>
+__device__ __forceinline__ T tb_ld(const T* lds, const uint8_t* vld, bool& ok, int r, int c) {
+  const int i = r * RW + c;
+  if (CHECK) ok = ok && vld[i];
+  return lds[i];
+}
+
+/* Bitwise OR of `v` over the workgroup (all work-items must call it). */
+__device__ __forceinline__ uint32_t block_or(uint32_t v) {
+  __shared__ uint32_t acc;
+  if (threadIdx.x == 0) acc = 0u;
+  __syncthreads();
+  if (v) atomicOr(&acc, v);
+  __syncthreads();
+  const uint32_t r = acc;
+  __syncthreads();
+  return r;
+}
+
 }  // namespace xg
 
 #endif

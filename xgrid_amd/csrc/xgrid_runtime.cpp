@@ -157,6 +157,15 @@ struct EventPair {
   hipEvent_t a, b;
 };
 
+struct LoopPlan {
+  xg_loop_desc d;
+  const void* grid = nullptr;   // the grid the plan was validated on
+  uint64_t mask_version = 0;    // ... and its mask
+  int state = 0;                // 1 exact, 0 unknown, -1 declined
+  int64_t launches = 0, iterations = 0;
+  std::vector<std::vector<uint8_t>> last_params;
+};
+
 }  // namespace
 
 struct xg_kernel {
@@ -179,6 +188,9 @@ struct xg_kernel {
   int64_t n_launch = 0, n_fused = 0, n_replay = 0, n_capture = 0;
   int32_t timed_sid = -1;
   std::vector<EventPair> events;
+  std::vector<LoopPlan> loops;
+  bool blocked = true;
+  int32_t* d_fail = nullptr;        // validation flag of blocked launches
   std::vector<int64_t> exec_count;  // stmt() calls per statement (all modes)
   std::vector<std::vector<uint8_t>> last_params;  // parameters of each statement's last call
   std::vector<uint8_t> key_buf;
@@ -649,7 +661,125 @@ int api_end(xg_kernel* k) {
 
 const int32_t* api_shape(xg_grid* g) { return g ? g->shape : nullptr; }
 
-const xg_api g_api = {api_begin, api_stmt, api_end, api_shape};
+// One launch of a blocked loop: `iters` iterations from level 0 into the scratch.
+int launch_blocked(xg_kernel* k, LoopPlan& L, xg_grid* G, const void* const* params, int iters, bool check) {
+  xg_tb_hdr TH;
+  std::memset(&TH, 0, sizeof TH);
+  fill_view(TH.src, G, G->levels[0]);
+  fill_view(TH.dst, G, G->scratch);
+  TH.mask = G->h_mask.empty() ? nullptr : G->d_mask;
+  TH.fail = k->d_fail;
+  TH.iters = iters;
+  TH.uniform = 1;
+  const int m = L.d.nstmts;
+  std::vector<std::vector<xg_gview>> refs((size_t)m);
+  std::vector<void*> args;
+  args.reserve(1 + 2 * (size_t)m);
+  args.push_back(&TH);
+  for (int j = 0; j < m; ++j) {
+    const xg_stmt_desc& d = k->stmts[(size_t)L.d.sids[j]].d;
+    TH.klass[j] = d.klass;
+    if (!uniform_refs(k, d)) TH.uniform = 0;
+    int rc = fill_refs(k, d, refs[(size_t)j]);
+    if (rc) return rc;
+  }
+  for (int j = 0; j < m; ++j) {
+    args.push_back(refs[(size_t)j].data());
+    args.push_back(const_cast<void*>(params[j]));
+  }
+  const int64_t blocks = ceil_div(G->shape[0], L.d.core0) * ceil_div(G->shape[1], L.d.core1);
+  L.launches++;
+  return do_launch(k, -1, check ? L.d.fn_check : L.d.fn, blocks, L.d.threads, args.data());
+}
+
+int api_loop(xg_kernel* k, int32_t lid, int64_t trip, const void* const* params, int32_t n) {
+  if (k->status != XG_OK) return 1;  // the call already failed: skip the loop
+  if (lid < 0 || lid >= (int32_t)k->loops.size() || n != k->loops[(size_t)lid].d.nstmts) {
+    set_status(k, fail(XG_ERR_ARG, "bad loop description"));
+    return 1;
+  }
+  LoopPlan& L = k->loops[(size_t)lid];
+  auto account = [&]() {
+    for (int j = 0; j < n; ++j) {
+      const int32_t sid = L.d.sids[j];
+      k->exec_count[(size_t)sid] += trip;
+      const int64_t b = k->stmts[(size_t)sid].d.param_bytes;
+      k->last_params[(size_t)sid].assign((const uint8_t*)params[j], (const uint8_t*)params[j] + b);
+    }
+    L.last_params.resize((size_t)n);
+    for (int j = 0; j < n; ++j) {
+      const int64_t b = k->stmts[(size_t)L.d.sids[j]].d.param_bytes;
+      L.last_params[(size_t)j].assign((const uint8_t*)params[j], (const uint8_t*)params[j] + b);
+    }
+  };
+  if (k->mode == kReplay) {  // the graph holds whatever ran when it was captured
+    account();
+    return 1;
+  }
+  if (!k->blocked || trip <= 0 || !L.d.fn) return 0;
+  xg_grid* G = k->grids[(size_t)L.d.grid_arg];
+  // another argument bound to the loop's grid would miss the LDS copy
+  for (int j = 0; j < n; ++j) {
+    const xg_stmt_desc& d = k->stmts[(size_t)L.d.sids[j]].d;
+    for (int r = 0; r < d.nrefs; ++r)
+      if (d.ref_arg[r] != L.d.grid_arg && k->grids[(size_t)d.ref_arg[r]] == G) return 0;
+  }
+  if (L.grid != (const void*)G || L.mask_version != G->mask_version) {
+    L.state = 0;
+    L.grid = G;
+    L.mask_version = G->mask_version;
+  }
+  if (L.state < 0) return 0;
+  if (L.state == 0 && k->mode == kCapture) return 0;  // validation needs a synchronisation
+  int rc = flush(k);
+  if (rc) {
+    set_status(k, rc);
+    return 1;
+  }
+  int64_t done = 0;
+  if (L.state == 0) {
+    if (!k->d_fail) {
+      void* p = nullptr;
+      if (hipMalloc(&p, sizeof(int32_t)) != hipSuccess) {
+        set_status(k, fail(XG_ERR_MEM, "hipMalloc failed"));
+        return 1;
+      }
+      k->d_fail = (int32_t*)p;
+    }
+    int32_t h_fail = 0;
+    const int T = (int)std::min<int64_t>(L.d.tmax, trip);
+    if (hipMemsetAsync(k->d_fail, 0, sizeof(int32_t), g_dev.stream) != hipSuccess ||
+        (rc = launch_blocked(k, L, G, params, T, true)) != XG_OK ||
+        hipMemcpyAsync(&h_fail, k->d_fail, sizeof(int32_t), hipMemcpyDeviceToHost, g_dev.stream) != hipSuccess ||
+        hipStreamSynchronize(g_dev.stream) != hipSuccess) {
+      set_status(k, rc ? rc : fail(XG_ERR_HIP, "blocked-loop validation failed"));
+      return 1;
+    }
+    if (h_fail) {  // some core cell depends on data outside its tile's halo: the scratch is discarded
+      L.state = -1;
+      return 0;
+    }
+    L.state = 1;
+    std::swap(G->levels[0], G->scratch);
+    done = T;
+    L.iterations += T;
+  }
+  while (done < trip) {
+    const int T = (int)std::min<int64_t>(L.d.tmax, trip - done);
+    rc = launch_blocked(k, L, G, params, T, false);
+    if (rc) {
+      set_status(k, rc);
+      return 1;
+    }
+    std::swap(G->levels[0], G->scratch);
+    done += T;
+    L.iterations += T;
+  }
+  account();
+  return 1;
+}
+
+const xg_api g_api = {api_begin, api_stmt, api_end, api_shape, api_loop};
 
 // copy probe kernel (calibration of HBM bandwidth and of byte counters)
 __global__ void __launch_bounds__(256) xg_copy_kernel(const double4* __restrict__ in, double4* __restrict__ out, int64_t n) {
@@ -940,8 +1070,8 @@ int xg_grid_class_info(xg_grid* g, int32_t klass, int32_t* kind, int64_t* count,
 }
 
 int xg_kernel_create(int32_t nstmts, const xg_stmt_desc* stmts, int32_t ngroups, const xg_group_desc* groups,
-                     int32_t ngrids, xg_kernel** out) {
-  if (nstmts < 0 || ngroups < 0 || ngrids < 0 || !out) return fail(XG_ERR_ARG, "invalid kernel description");
+                     int32_t nloops, const xg_loop_desc* loops, int32_t ngrids, xg_kernel** out) {
+  if (nstmts < 0 || ngroups < 0 || nloops < 0 || ngrids < 0 || !out) return fail(XG_ERR_ARG, "invalid kernel description");
   auto* k = new xg_kernel;
   k->ngrids = ngrids;
   for (int i = 0; i < nstmts; ++i) {
@@ -967,6 +1097,22 @@ int xg_kernel_create(int32_t nstmts, const xg_stmt_desc* stmts, int32_t ngroups,
     }
     k->groups.push_back(groups[i]);
   }
+  for (int i = 0; i < nloops; ++i) {
+    const xg_loop_desc& d = loops[i];
+    bool ok = d.nstmts >= 1 && d.nstmts <= XG_LOOP_MAX_STMTS && d.grid_arg >= 0 && d.grid_arg < ngrids &&
+              d.core0 > 0 && d.core1 > 0 && d.tmax >= 1 && d.halo >= 0 && d.threads >= 64 &&
+              d.threads <= 1024 && d.threads % 64 == 0;
+    for (int j = 0; ok && j < d.nstmts; ++j)
+      ok = d.sids[j] >= 0 && d.sids[j] < nstmts && k->stmts[(size_t)d.sids[j]].d.store_arg == d.grid_arg &&
+           k->stmts[(size_t)d.sids[j]].d.dims == 2;
+    if (!ok) {
+      delete k;
+      return fail(XG_ERR_ARG, "invalid loop description " + std::to_string(i));
+    }
+    LoopPlan L;
+    L.d = d;
+    k->loops.push_back(L);
+  }
   k->exec_count.assign((size_t)nstmts, 0);
   k->last_params.assign((size_t)nstmts, std::vector<uint8_t>());
   *out = k;
@@ -980,6 +1126,7 @@ int xg_kernel_destroy(xg_kernel* k) {
     (void)hipEventDestroy(ev.a);
     (void)hipEventDestroy(ev.b);
   }
+  if (k->d_fail) (void)hipFree(k->d_fail);
   delete k;
   return XG_OK;
 }
@@ -990,15 +1137,64 @@ int xg_kernel_status(xg_kernel* k) {
   return k->status;
 }
 
-int xg_kernel_set_options(xg_kernel* k, int32_t graphs, int32_t fuse) {
+int xg_kernel_set_options(xg_kernel* k, int32_t graphs, int32_t fuse, int32_t blocked) {
   if (!k) return fail(XG_ERR_ARG, "null kernel");
   k->graphs = graphs != 0;
   k->fuse = fuse != 0;
-  if (!k->graphs) {
-    for (auto& e : k->cache) destroy_entry(e);
-    k->cache.clear();
-  }
+  k->blocked = blocked != 0;
+  // captured graphs embody the old options
+  for (auto& e : k->cache) destroy_entry(e);
+  k->cache.clear();
   return XG_OK;
+}
+
+int xg_kernel_loop_info(xg_kernel* k, int32_t lid, int64_t* launches, int64_t* iterations, int32_t* state) {
+  if (!k || lid < 0 || lid >= (int32_t)k->loops.size()) return fail(XG_ERR_ARG, "invalid loop");
+  const LoopPlan& L = k->loops[(size_t)lid];
+  if (launches) *launches = L.launches;
+  if (iterations) *iterations = L.iterations;
+  if (state) *state = L.state;
+  return XG_OK;
+}
+
+int xg_kernel_time_loop(xg_kernel* k, int32_t lid, int32_t trip, int32_t reps, double* ms_per_launch) {
+  if (int rc = check_ready()) return rc;
+  if (!k || lid < 0 || lid >= (int32_t)k->loops.size() || reps < 1 || trip < 1) return fail(XG_ERR_ARG, "invalid arguments");
+  LoopPlan& L = k->loops[(size_t)lid];
+  if (L.state != 1 || L.last_params.size() != (size_t)L.d.nstmts || k->grids.empty())
+    return fail(XG_ERR_STATE, "the loop has not run blocked yet");
+  if (k->mode != kDirect) return fail(XG_ERR_STATE, "a call is in progress");
+  xg_grid* G = k->grids[(size_t)L.d.grid_arg];
+  std::vector<const void*> ps;
+  for (auto& p : L.last_params) ps.push_back(p.data());
+  // one trip = the launches a call makes for `trip` iterations (chunks of tmax)
+  auto run_trip = [&]() -> int {
+    for (int done = 0; done < trip;) {
+      const int T = std::min(L.d.tmax, trip - done);
+      int rc = launch_blocked(k, L, G, ps.data(), T, false);
+      if (rc) return rc;
+      std::swap(G->levels[0], G->scratch);
+      done += T;
+    }
+    return XG_OK;
+  };
+  const int64_t per_trip = ceil_div(trip, L.d.tmax);
+  hipEvent_t e0, e1;
+  XG_HIP(hipEventCreate(&e0));
+  XG_HIP(hipEventCreate(&e1));
+  int rc = run_trip();
+  if (!rc) {
+    XG_HIP(hipEventRecord(e0, g_dev.stream));
+    for (int i = 0; i < reps && !rc; ++i) rc = run_trip();
+    XG_HIP(hipEventRecord(e1, g_dev.stream));
+    XG_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    XG_HIP(hipEventElapsedTime(&ms, e0, e1));
+    if (ms_per_launch) *ms_per_launch = (double)ms / ((double)reps * (double)per_trip);
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return rc;
 }
 
 int xg_kernel_counters(xg_kernel* k, int64_t* launches, int64_t* fused, int64_t* replays, int64_t* captures) {

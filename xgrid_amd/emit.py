@@ -91,6 +91,21 @@ class GroupMeta:
 
 
 @dataclass
+class LoopMeta:
+    """A `for` loop that may run temporally blocked (several iterations per launch)."""
+    lid: int
+    node: object                  # ir.For
+    stmts: list                   # StmtMeta of the body, in order
+    grid: str                     # the one grid the body stores to
+    core: tuple                   # (rows, columns) of a workgroup's core tile
+    halo: int
+    tmax: int
+    kernel: str
+    threads: int = 512
+    reach: int = 1                # largest |offset| of the body's reads of the loop grid
+
+
+@dataclass
 class Lowered:
     device_src: str
     stub_src: str
@@ -99,6 +114,7 @@ class Lowered:
     grid_args: list               # [(arg index, name, GridType)]
     scalar_args: list             # [(arg index, name, type)]
     graphs_ok: bool
+    loops: list = field(default_factory=list)
 
 
 class _ExprC:
@@ -186,6 +202,7 @@ class Emitter:
         self.entry = entry
         self.functions = []            # user operators reachable from the kernel (dependency order)
         self.device_functions = []     # those reachable from stencil expressions
+        self._prefetched = set()       # blocked-loop statements whose other-grid loads are prefetched
         self._collect_functions()
 
     # ------------------------------------------------------------- helpers
@@ -241,13 +258,23 @@ class Emitter:
             raise EmitError(f"{st.node.loc}: a statement may load at most 16 distinct (grid, time level) arrays")
         return refs
 
-    def _device_stmt(self, st: StencilStmt, refs: list, params: list, out: list):
+    def _device_stmt(self, st: StencilStmt, refs: list, params: list, out: list, tb: LoopMeta | None = None):
         sid = st.sid
         D = st.dimension
         T = device_type_name(st.gtype.element)
         grids = {n: t for _, n, t in self.func.grid_args}
+        in_lds = [False]     # while lowering value_tb: the loop grid's level 0 lives in LDS
+        pre_mode = [False]   # ... and the other loads come from a per-cell prefetch
+        pre_slots: list = []  # (grid, level, offsets) of each prefetched load
 
         def cell(e: ir.Cell) -> str:
+            if in_lds[0] and e.grid.name == st.grid and e.level == 0:
+                return f"xg::tb_ld<T, CHECK, RHH, RWW>(lds, vld, ok, lr + {e.offsets[0]}, lc + {e.offsets[1]})"
+            if pre_mode[0]:
+                key = (e.grid.name, e.level, tuple(e.offsets))
+                if key not in pre_slots:
+                    pre_slots.append(key)
+                return f"pre.v{pre_slots.index(key)}"
             r = refs.index((e.grid.name, e.level))
             gt = grids[e.grid.name]
             if gt.dimension > D:
@@ -265,6 +292,17 @@ class Emitter:
             return f"R.v[{r}].shape[{_ExprC(True, cell, name, grid_info)(e.dim)}]"
 
         expr = _ExprC(True, cell, name, grid_info)(st.node.value)
+        expr_tb = expr_pre = None
+        if tb is not None:
+            in_lds[0] = True
+            expr_tb = _ExprC(True, cell, name, grid_info)(st.node.value)
+            if st.klass == 0:
+                pre_mode[0] = True
+                expr_pre = _ExprC(True, cell, name, grid_info)(st.node.value)
+                pre_mode[0] = False
+                if len(pre_slots) > self.PREFETCH_MAX:
+                    expr_pre = None
+            in_lds[0] = False
         so = list(st.store_offsets) + [0] * (3 - len(st.store_offsets))
         rl, rh = [0, 0, 0], [0, 0, 0]
         for n in ir.walk(st.node.value):
@@ -292,6 +330,41 @@ class Emitter:
         out.append("    (void)R; (void)P; (void)sh; (void)xc; (void)i0; (void)i1; (void)i2;")
         out.append(f"    return {expr};")
         out.append("  }")
+        if expr_tb is not None:
+            # the same expression with the loop grid's level 0 read from the workgroup's LDS region
+            out.append("  template <bool SAFE, bool CHECK, int RHH, int RWW>")
+            out.append("  __device__ static __forceinline__ T value_tb(const T* lds, const uint8_t* vld, bool& ok, "
+                       "int lr, int lc, const Refs& R, const Params& P, const int32_t* sh, int i0, int i1, int i2) {")
+            out.append(f"    const int64_t xc = SAFE ? xg::lin<{D}>(sh, i0, i1, i2) : 0;")
+            out.append("    (void)lds; (void)vld; (void)ok; (void)lr; (void)lc; (void)R; (void)P; (void)sh; (void)xc;"
+                       " (void)i0; (void)i1; (void)i2;")
+            out.append(f"    return {expr_tb};")
+            out.append("  }")
+        if expr_pre is not None:
+            # loads of grids the loop does not write, fetched once per launch per cell
+            gts = {n: t for _, n, t in self.func.grid_args}
+            out.append(f"  static constexpr int NPRE = {len(pre_slots)};")
+            fields_ = " ".join(f"{device_type_name(gts[g].element)} v{i};" for i, (g, _, _) in enumerate(pre_slots))
+            out.append(f"  struct Pre {{ {fields_ or 'int32_t unused;'} }};")
+            out.append("  template <bool SAFE>")
+            out.append("  __device__ static __forceinline__ void prefetch(Pre& pre, const Refs& R, const int32_t* sh, "
+                       "int i0, int i1, int i2) {")
+            out.append(f"    const int64_t xc = SAFE ? xg::lin<{D}>(sh, i0, i1, i2) : 0;")
+            out.append("    (void)R; (void)sh; (void)xc; (void)pre;")
+            for i, (g, lvl, offs) in enumerate(pre_slots):
+                gt = gts[g]
+                r = refs.index((g, lvl))
+                o3 = [str(offs[d]) if d < gt.dimension else "0" for d in range(3)]
+                out.append(f"    pre.v{i} = xg::ld<{device_type_name(gt.element)}, {gt.dimension}, XG_OVERSTEP, SAFE>"
+                           f"(R.v[{r}], sh, xc, {', '.join(o3)}, i0, i1, i2);")
+            out.append("  }")
+            out.append("  template <bool CHECK, int RHH, int RWW>")
+            out.append("  __device__ static __forceinline__ T value_pre(const T* lds, const uint8_t* vld, bool& ok, "
+                       "int lr, int lc, const Refs& R, const Params& P, const Pre& pre) {")
+            out.append("    (void)lds; (void)vld; (void)ok; (void)lr; (void)lc; (void)R; (void)P; (void)pre;")
+            out.append(f"    return {expr_pre};")
+            out.append("  }")
+            self._prefetched.add(sid)
         out.append("};")
         args = f"const xg_launch_hdr H, const xg_r{sid} R, const xg_p{sid} P"
         out.append(f'extern "C" __global__ void __launch_bounds__(256) xg_k{sid}({args}) '
@@ -311,9 +384,204 @@ class Emitter:
         self._stmts_text(f.body, out, expr, 1, device=True)
         out.append("}")
 
-    def device_source(self, stmt_meta: list, groups: list) -> str:
+    def _tb_kernel(self, L: LoopMeta, out: list):
+        """Blocked kernel of one loop: each workgroup keeps core + halo of the
+        loop grid in LDS and runs up to `tmax` iterations of the body on it.
+        Every statement is computed for all its cells in the region (reading
+        the region as it was before the statement), then written back, so a
+        statement sees exactly the values the statements before it produced."""
+        T = device_type_name(L.stmts[0].stmt.gtype.element)
+        c0, c1 = L.core
+        m = len(L.stmts)
+        params = ["const xg_tb_hdr& TH"] + [f"const xg_r{s.stmt.sid}& R{j}, const xg_p{s.stmt.sid}& P{j}"
+                                            for j, s in enumerate(L.stmts)]
+        # reach of the loads that come from global memory (other grids / levels)
+        rl, rh = [0, 0], [0, 0]
+        for s in L.stmts:
+            for n in ir.walk(s.stmt.node.value):
+                if isinstance(n, ir.Cell) and not (n.grid.name == L.grid and n.level == 0):
+                    for d in range(2):
+                        rl[d] = max(rl[d], -n.offsets[d])
+                        rh[d] = max(rh[d], n.offsets[d])
+        o = out.append
+        o(f"template <bool CHECK>")
+        o(f"__device__ __forceinline__ void {L.kernel}_body({', '.join(params)}) {{")
+        o(f"  typedef {T} T;")
+        o(f"  constexpr int TBT = {L.threads};")
+        o(f"  constexpr int C0 = {c0}, C1 = {c1}, HL = {L.halo}, RHH = C0 + 2 * HL, RWW = C1 + 2 * HL;")
+        o("  constexpr int NR = RHH * RWW, NC = (NR + TBT - 1) / TBT;")
+        # two copies of the region (each statement reads one and writes the other), the
+        # statement-membership bits of every cell and, when validating, exactness bytes
+        o("  __shared__ T buf[2 * NR];")
+        o("  __shared__ uint8_t mem[NR];")
+        o("  __shared__ uint8_t vl[CHECK ? 2 * NR : 1];")
+        # cells of the boundary (class != 0) statements, listed once per launch
+        sparse_mask = sum(1 << j for j, s in enumerate(L.stmts) if s.stmt.klass != 0)
+        o(f"  constexpr uint32_t SPARSE = {sparse_mask}u;")
+        o("  constexpr int SL = NR / 4 > 256 ? NR / 4 : (NR < 256 ? NR : 256);   /* overflow: plan declined */")
+        o("  __shared__ uint16_t slist[SL];")
+        o("  __shared__ int scount;")
+        o("  if (threadIdx.x == 0) scount = 0;")
+        o("  __syncthreads();")
+        o("  const int32_t* sh = TH.src.shape;")
+        o("  const int N0 = sh[0], N1 = sh[1];")
+        o("  const int nt1 = (N1 + C1 - 1) / C1;")
+        o("  const int tile = (int)xg::xcd_remap(blockIdx.x, gridDim.x);")
+        o("  const int g0 = (tile / nt1) * C0, g1 = (tile % nt1) * C1;")
+        o("  const int r0 = g0 - HL, r1 = g1 - HL;")
+        o("  const T* __restrict__ src = static_cast<const T*>(TH.src.ptr);")
+        o("  uint32_t any = 0;")
+        o("#pragma unroll")
+        o("  for (int n = 0; n < NC; ++n) {")
+        o("    const int idx = threadIdx.x + TBT * n;")
+        o("    if (idx < NR) {")
+        o("      const int i0 = r0 + idx / RWW, i1 = r1 + idx % RWW;")
+        o("      const bool in = i0 >= 0 && i0 < N0 && i1 >= 0 && i1 < N1;")
+        o("      const int64_t c = (int64_t)i0 * N1 + i1;")
+        o("      buf[idx] = in ? src[c] : T(0);")
+        o("      uint32_t mb = 0u;")
+        o("      if (in) {")
+        o("        const int32_t k = TH.mask ? TH.mask[c] : 0;")
+        o("        mb = " + " | ".join(f"(k == TH.klass[{j}] ? {1 << j}u : 0u)" for j in range(m)) + ";")
+        o("      }")
+        o("      mem[idx] = (uint8_t)mb;")
+        o("      any |= mb;")
+        o("      if (mb & SPARSE) {")
+        o("        const int pos = atomicAdd(&scount, 1);")
+        o("        if (pos < SL) slist[pos] = (uint16_t)idx;")
+        o("      }")
+        o("      if (CHECK) vl[idx] = in ? 1 : 0;")
+        o("    }")
+        o("  }")
+        o("  any = xg::block_or(any);")
+        o("  const int ns = scount < SL ? scount : SL;")
+        o("  if (CHECK && threadIdx.x == 0 && scount > SL) *TH.fail = 1;   /* list overflow: decline */")
+        o(f"  const bool safe = TH.uniform && r0 - {rl[0]} >= 0 && r0 + RHH + {rh[0]} <= N0 && "
+          f"r1 - {rl[1]} >= 0 && r1 + RWW + {rh[1]} <= N1;")
+        # Cells at least `reach` inside the region are computed; their reads stay in the
+        # region, so LDS indices need no clamp.  Cells nearer the edge only feed them
+        # (and, being stale, are marked not exact by the validation variant).
+        o(f"  constexpr int RE = {L.reach};")
+        o("  int cb = 0;                 /* offset of the current copy in buf / vl */")
+        # one instantiation per addressing mode of the other grids, chosen once per workgroup
+        o("  auto iterate = [&](auto safe_c) {")
+        o("  constexpr bool SAFE = decltype(safe_c)::value;")
+        for j, s in enumerate(L.stmts):
+            if s.stmt.sid in self._prefetched:
+                S = f"xg_s{s.stmt.sid}"
+                o(f"  typename {S}::Pre pre{j}[NC];")
+                o("#pragma unroll")
+                o("  for (int k = 0; k < NC; ++k) {")
+                o("    const int idx = threadIdx.x + TBT * k;")
+                o(f"    if (idx < NR && (mem[idx] & {1 << j}u))")
+                o(f"      {S}::template prefetch<SAFE>(pre{j}[k], R{j}, sh, r0 + idx / RWW, r1 + idx % RWW, 0);")
+                o("  }")
+        o("  for (int it = 0; it < TH.iters; ++it) {")
+        for j, s in enumerate(L.stmts):
+            S = f"xg_s{s.stmt.sid}"
+            bit = f"{1 << j}u"
+            if s.stmt.klass != 0:
+                # boundary statement: its listed cells into the spare copy, then back
+                o(f"    if (any & {bit}) {{")
+                o("      T* cur = buf + cb;")
+                o("      T* spare = buf + (NR - cb);")
+                o("      uint8_t* vcur = vl + (CHECK ? cb : 0);")
+                o("      uint8_t* vsp = vl + (CHECK ? NR - cb : 0);")
+                o("      for (int e = threadIdx.x; e < ns; e += TBT) {")
+                o("        const int idx = slist[e];")
+                o(f"        if (mem[idx] & {bit}) {{")
+                o("          const int lr = idx / RWW, lc = idx % RWW;")
+                o("          if (lr >= RE && lr < RHH - RE && lc >= RE && lc < RWW - RE) {")
+                o("            bool ok = true;")
+                o(f"            spare[idx] = {S}::template value_tb<SAFE, CHECK, RHH, RWW>(cur, vcur, ok, lr, lc, "
+                  f"R{j}, P{j}, sh, r0 + lr, r1 + lc, 0);")
+                o("            if (CHECK) vsp[idx] = ok ? 1 : 0;")
+                o("          } else if (CHECK) {")
+                o("            vsp[idx] = 0;          /* too near the region edge: stays stale */")
+                o("            spare[idx] = cur[idx];")
+                o("          } else {")
+                o("            spare[idx] = cur[idx];")
+                o("          }")
+                o("        }")
+                o("      }")
+                o("      __syncthreads();")
+                o("      for (int e = threadIdx.x; e < ns; e += TBT) {")
+                o("        const int idx = slist[e];")
+                o(f"        if (mem[idx] & {bit}) {{")
+                o("          cur[idx] = spare[idx];")
+                o("          if (CHECK) vcur[idx] = vsp[idx];")
+                o("        }")
+                o("      }")
+                o("      __syncthreads();")
+                o("    }")
+                continue
+            o(f"    if (any & {bit}) {{")
+            o("      const T* cur = buf + cb;")
+            o("      T* nxt = buf + (NR - cb);")
+            o("      const uint8_t* vcur = vl + (CHECK ? cb : 0);")
+            o("      uint8_t* vnxt = vl + (CHECK ? NR - cb : 0);")
+            # after `it` iterations only cells (it + 1) * reach inside the region can still be
+            # exact: nearer ones are left stale (the validation variant marks them)
+            o("      const int lo = RE * (it + 1) < RHH / 2 ? RE * (it + 1) : RHH / 2;")
+            pre = s.stmt.sid in self._prefetched
+            if pre:
+                o("#pragma unroll")
+                o("      for (int k = 0; k < NC; ++k) {")
+                o("        const int idx = threadIdx.x + TBT * k;")
+                o("        if (idx >= NR) continue;")
+            else:
+                o("#pragma unroll 2")
+                o("      for (int idx = threadIdx.x; idx < NR; idx += TBT) {")
+            o("        const int lr = idx / RWW, lc = idx % RWW;")
+            o(f"        const bool mine = (mem[idx] & {bit}) != 0;")
+            o("        const bool act = lr >= lo && lr < RHH - lo && lc >= lo && lc < RWW - lo;")
+            o("        if (mine && act) {")
+            o("          bool ok = true;")
+            if pre:
+                o(f"          nxt[idx] = {S}::template value_pre<CHECK, RHH, RWW>(cur, vcur, ok, lr, lc, R{j}, P{j}, "
+                  f"pre{j}[k]);")
+            else:
+                o(f"          nxt[idx] = {S}::template value_tb<SAFE, CHECK, RHH, RWW>(cur, vcur, ok, lr, lc, R{j}, "
+                  f"P{j}, sh, r0 + lr, r1 + lc, 0);")
+            o("          if (CHECK) vnxt[idx] = ok ? 1 : 0;")
+            o("        } else {")
+            o("          nxt[idx] = cur[idx];")
+            o("          if (CHECK) vnxt[idx] = mine ? 0 : vcur[idx];   /* a stale member is not exact */")
+            o("        }")
+            o("      }")
+            o("      __syncthreads();")
+            o("      cb = NR - cb;")
+            o("    }")
+        o("  }")
+        o("  };")
+        o("  if (safe) iterate(xg::bconst<true>()); else iterate(xg::bconst<false>());")
+        o("  const T* lds = buf + cb;")
+        o("  const uint8_t* vld = vl + (CHECK ? cb : 0);")
+        o("  T* __restrict__ dst = static_cast<T*>(TH.dst.ptr);")
+        o("  constexpr int NCORE = (C0 * C1 + TBT - 1) / TBT;")
+        o("#pragma unroll")
+        o("  for (int n = 0; n < NCORE; ++n) {")
+        o("    const int q = threadIdx.x + TBT * n;")
+        o("    const int i0 = g0 + q / C1, i1 = g1 + q % C1;")
+        o("    if (q < C0 * C1 && i0 < N0 && i1 < N1) {")
+        o("      const int li = (HL + q / C1) * RWW + HL + q % C1;")
+        o("      dst[(int64_t)i0 * N1 + i1] = lds[li];")
+        o("      if (CHECK && !vld[li]) *TH.fail = 1;")
+        o("    }")
+        o("  }")
+        o("}")
+        kargs = ["const xg_tb_hdr TH"] + [f"const xg_r{s.stmt.sid} R{j}, const xg_p{s.stmt.sid} P{j}"
+                                          for j, s in enumerate(L.stmts)]
+        call = ", ".join(["TH"] + [f"R{j}, P{j}" for j in range(m)])
+        o(f'extern "C" __global__ void __launch_bounds__({L.threads}) {L.kernel}({", ".join(kargs)}) '
+          f"{{ {L.kernel}_body<false>({call}); }}")
+        o(f'extern "C" __global__ void __launch_bounds__({L.threads}) {L.kernel}_chk({", ".join(kargs)}) '
+          f"{{ {L.kernel}_body<true>({call}); }}")
+
+    def device_source(self, stmt_meta: list, groups: list, loops: list = ()) -> str:
         out = ["// generated by xgrid_amd for the MI355X backend", '#include "xgrid_device.h"',
                f"#define XG_OVERSTEP {OVERSTEP_CODE[self.overstep]}"]
+        loop_of = {m.stmt.sid: L for L in loops for m in L.stmts}
         for s in self._structs():
             fields_ = " ".join(f"{device_type_name(t)} {n};" for n, t in s.elements)
             out.append(f"struct {s.name} {{ {fields_} }};")
@@ -324,7 +592,9 @@ class Emitter:
         for op in self.device_functions:
             self._device_function(op, out)
         for m in stmt_meta:
-            self._device_stmt(m.stmt, m.refs, m.params, out)
+            self._device_stmt(m.stmt, m.refs, m.params, out, loop_of.get(m.stmt.sid))
+        for L in loops:
+            self._tb_kernel(L, out)
         # phase-2 kernels per (element type, dimension)
         done = set()
         for m in stmt_meta:
@@ -375,6 +645,27 @@ class Emitter:
                 out.append(f"{ind}while ({expr(s.cond)}) {{")
                 self._stmts_text(s.body, out, expr, depth + 1, device, stencil)
                 out.append(f"{ind}}}")
+            elif isinstance(s, ir.For) and not device and s in getattr(self, "_stub_loops", {}):
+                # offer the whole loop to the runtime (blocked); run it here if declined
+                L = self._stub_loops[s]
+                v = s.var.name
+                ct = ctype_name(s.var.type)
+                out.append(f"{ind}{{")
+                out.append(f"{ind}  {ct} xg_s = {expr(s.start)}, xg_e = {expr(s.stop)}, xg_st = {expr(s.step)};")
+                out.append(f"{ind}  int64_t xg_n = (xg_st > 0 && xg_e > xg_s) ? (((int64_t)xg_e - (int64_t)xg_s + "
+                           f"(int64_t)xg_st - 1) / (int64_t)xg_st) : 0;")
+                for j, m in enumerate(L.stmts):
+                    out.append(f"{ind}  struct xg_p{m.stmt.sid} xg_L{j}; memset(&xg_L{j}, 0, sizeof xg_L{j}); "
+                               + self._param_sets(m, f"xg_L{j}"))
+                out.append(f"{ind}  const void* xg_LP[{len(L.stmts)}] = {{{', '.join(f'&xg_L{j}' for j in range(len(L.stmts)))}}};")
+                out.append(f"{ind}  if (xg_st > 0 && xg_A->loop(xg_K, {L.lid}, xg_n, xg_LP, {len(L.stmts)})) {{")
+                out.append(f"{ind}    {v} = xg_n > 0 ? ({ct})(xg_s + ({ct})xg_n * xg_st) : xg_s;")
+                out.append(f"{ind}  }} else {{")
+                out.append(f"{ind}    for ({v} = xg_s; {v} < {expr(s.stop)}; {v} += {expr(s.step)}) {{")
+                self._stmts_text(s.body, out, expr, depth + 3, device, stencil)
+                out.append(f"{ind}    }}")
+                out.append(f"{ind}  }}")
+                out.append(f"{ind}}}")
             elif isinstance(s, ir.For):
                 v = s.var.name
                 out.append(f"{ind}for ({v} = {expr(s.start)}; {v} < {expr(s.stop)}; {v} += {expr(s.step)}) {{")
@@ -394,8 +685,18 @@ class Emitter:
                 raise EmitError(f"cannot lower statement {type(s).__name__}")
 
     # ------------------------------------------------------------- host stub
-    def stub_source(self, stmt_meta: list) -> str:
+    @staticmethod
+    def _param_sets(m, var: str) -> str:
+        """C assignments filling a statement's parameter block `var`."""
+        sets = []
+        for v in m.params:
+            src = f"(*{v.name})" if isinstance(v.type, Pointer) else v.name
+            sets.append(f"{var}.{v.name} = {src};")
+        return " ".join(sets)
+
+    def stub_source(self, stmt_meta: list, loops: list = ()) -> str:
         f = self.func
+        self._stub_loops = {L.node: L for L in loops}
         out = ["/* generated by xgrid_amd: host control flow of one kernel */",
                "#include <stdio.h>", "#include <stdlib.h>", "#include <stdint.h>", "#include <stdbool.h>",
                "#include <string.h>", "#include <math.h>"]
@@ -403,7 +704,8 @@ class Emitter:
             out.append(f'#include "{inc}"')
         out.append("typedef struct xg_api_t { int (*begin)(void*, void* const*, int32_t, const void*, int64_t);"
                    " int (*stmt)(void*, int32_t, const void*, int64_t); int (*end)(void*);"
-                   " const int32_t* (*shape)(void*); } xg_api_t;")
+                   " const int32_t* (*shape)(void*);"
+                   " int (*loop)(void*, int32_t, int64_t, const void* const*, int32_t); } xg_api_t;")
         for s in self._structs():
             fields_ = " ".join(f"{ctype_name(t)} {n};" for n, t in s.elements)
             out.append(f"struct {s.name} {{ {fields_} }};")
@@ -436,12 +738,8 @@ class Emitter:
 
         def stencil(s: ir.Assign) -> str:
             m = next(mm for mm in stmt_meta if mm.stmt.node is s)
-            sets = []
-            for v in m.params:
-                src = f"(*{v.name})" if isinstance(v.type, Pointer) else v.name
-                sets.append(f"xg_P.{v.name} = {src};")
             return ("{ struct xg_p%d xg_P; memset(&xg_P, 0, sizeof xg_P); %s xg_A->stmt(xg_K, %d, &xg_P, sizeof xg_P); }"
-                    % (m.stmt.sid, " ".join(sets), m.stmt.sid))
+                    % (m.stmt.sid, self._param_sets(m, "xg_P"), m.stmt.sid))
 
         def param_decl(n, t):
             if isinstance(t, GridType):
@@ -520,12 +818,64 @@ class Emitter:
                                       f"xg_k{st.sid}_pp" if st.mode == "pingpong" else None,
                                       phase2_names[key]))
         groups = self._groups(stmt_meta)
+        loops = self._blockable_loops(stmt_meta)
         graphs_ok = not any(isinstance(n, ir.Inline) for n in ir.walk_all(self.func.body)) and \
             not any(op.mode == "external" for op in self.functions)
-        return Lowered(self.device_source(stmt_meta, groups), self.stub_source(stmt_meta), stmt_meta, groups,
-                       list(self.func.grid_args),
+        return Lowered(self.device_source(stmt_meta, groups, loops), self.stub_source(stmt_meta, loops), stmt_meta,
+                       groups, list(self.func.grid_args),
                        [(i, n, t) for i, (n, t) in enumerate(self.func.args) if not isinstance(t, GridType)],
-                       graphs_ok)
+                       graphs_ok, loops)
+
+    PREFETCH_MAX = 4   # most other-grid loads of a blocked statement kept in registers per cell
+
+    @staticmethod
+    def block_shape() -> tuple:
+        """(core rows, region columns, work-items, most iterations per launch) of blocked
+        loops.  The region (core + halo) is a power of two wide so a region index splits
+        into row and column with shifts; the core is the region width less the halo.
+        XGRID_TB_SHAPE="rows,region_cols,threads,tmax" overrides the default."""
+        import os
+        spec = os.environ.get("XGRID_TB_SHAPE", "48,64,768,10")
+        r, c, t, m = (int(x) for x in spec.split(","))
+        return r, c, t, m
+
+    def _blockable_loops(self, stmt_meta: list) -> list:
+        """`for` loops that may run temporally blocked: the body is 1..8 stencil
+        statements, all storing unshifted to one 2-D grid, none two-phase, none
+        capturing the loop variable; integer loop variable; overstep 'none'."""
+        if self.overstep != "none":
+            return []
+        by_node = {m.stmt.node: m for m in stmt_meta}
+        loops = []
+        for n in ir.walk_all(self.func.body):
+            if not isinstance(n, ir.For) or not isinstance(n.var.type, Integer):
+                continue
+            if not n.body or len(n.body) > 8 or not all(isinstance(s, ir.Assign) and s in by_node for s in n.body):
+                continue
+            ms = [by_node[s] for s in n.body]
+            g = ms[0].stmt.grid
+            if any(m.stmt.grid != g or m.stmt.dimension != 2 or m.stmt.mode == "twophase" or any(m.stmt.store_offsets)
+                   for m in ms):
+                continue
+            if any(any(v is n.var for v in m.params) for m in ms):
+                continue
+            reach = 0
+            for m in ms:
+                for c in ir.walk(m.stmt.node.value):
+                    if isinstance(c, ir.Cell) and c.grid.name == g and c.level == 0:
+                        reach = max(reach, max(abs(o) for o in c.offsets))
+            rows, region_cols, threads, tm = self.block_shape()
+            tmax = max(1, tm // max(1, reach)) if reach else tm
+            # information moves `reach` cells per iteration through the bulk; a boundary
+            # statement reading a neighbour computed in the same iteration adds a hop,
+            # so leave room for two such hops (a validation launch proves the choice)
+            halo = tmax * reach + 2 * reach
+            cols = region_cols - 2 * halo
+            if halo > 16 or cols < 16:
+                continue
+            loops.append(LoopMeta(len(loops), n, ms, g, (rows, cols), halo, tmax, f"xg_tb{len(loops)}", threads,
+                                  reach))
+        return loops
 
     def _groups(self, stmt_meta: list) -> list:
         """Runs of consecutive stencil statements (same block, nothing between)."""
