@@ -42,10 +42,15 @@ def main():
     ap.add_argument("--instrumented-steps", type=int, default=20)
     ap.add_argument("--no-graphs", action="store_true")
     ap.add_argument("--no-fuse", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run the device copy kernel on MIB MiB (byte-counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this backend runs on one GPU; multi-GPU runs are not part of this build"}))
         sys.exit(2)
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
 
     import numpy as np
     import xgrid_amd
@@ -103,6 +108,7 @@ def main():
     sweep_ms = native.time_isolated(jac, reps)
     avg_ms, bytes_per_launch = sweep_ms, jac_bytes
     kernel_name = f"xg_k{jac}_pp (Jacobi sweep)"
+    kernel_id = f"xg_k{jac}_pp:n{n}"
     loop = native.loop_info(0) if native.lowered.loops else None
     if loop and loop["state"] == 1:
         # the pressure loop runs temporally blocked: its kernel dominates the step
@@ -118,6 +124,7 @@ def main():
         kernel_name = (f"{L.kernel} (pressure loop blocked: {trip} iterations of Jacobi sweep + 4 boundary "
                        f"statements in {launches_per_trip} launches of <= {L.tmax}, {L.core[0]}x{L.core[1]} "
                        f"tiles, halo {L.halo})")
+        kernel_id = f"{L.kernel}:{L.core[0]}x{L.core[1]}:h{L.halo}:t{L.tmax}:n{n}"
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
 
     name = ctypes_name(lib)
@@ -138,7 +145,8 @@ def main():
                                "plain @xgrid.kernel calls",
                    "grid": [n, n], "sweeps_per_step": 51, "device": name},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(kernel_id),
+                     "kernel_id": kernel_id,
                      "kernel": kernel_name, "bytes_per_launch": bytes_per_launch,
                      "avg_launch_ms": avg_ms},
         "jacobi_sweep": {"kernel": f"xg_k{jac}_pp", "bytes_per_launch": jac_bytes, "avg_launch_ms": sweep_ms,
@@ -153,6 +161,30 @@ def main():
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(n, args.dt, args.cpu_seconds)
     print(json.dumps(out), flush=True)
+
+
+def copy_probe(mib: int, iters: int = 20) -> None:
+    """Copy `mib` MiB on the device `iters` times (read + write of every byte)."""
+    import ctypes
+    from xgrid_amd import _lib
+    lib = _lib.ensure_device(0)
+    g, ms = ctypes.c_double(), ctypes.c_double()
+    _lib.check(lib.xg_copy_probe(mib << 20, iters, ctypes.byref(g), ctypes.byref(ms)))
+    print(json.dumps({"copy_probe_MiB": mib, "iters": iters, "GBs": g.value, "ms_per_copy": ms.value,
+                      "bytes_read_per_copy": mib << 20, "bytes_written_per_copy": mib << 20}), flush=True)
+
+
+def measured_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter summary
+    (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration), or None."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            data = json.load(f)
+    except (OSError, ValueError):
+        return None
+    entry = data.get("kernels", {}).get(kernel)
+    return None if entry is None else entry.get("bytes_per_launch")
 
 
 def ctypes_name(lib) -> str:
