@@ -44,6 +44,8 @@ def main():
     ap.add_argument("--no-fuse", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run the device copy kernel on MIB MiB (byte-counter calibration)")
+    ap.add_argument("--legs", default="all",
+                    help="secondary workloads after the headline one: all, none, or a comma list")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this backend runs on one GPU; multi-GPU runs are not part of this build"}))
@@ -158,9 +160,105 @@ def main():
                           "frac": step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS},
         "counters": native.counters(),
     }
+    if args.legs != "none":
+        legs = {}
+        wanted = set(LEGS) if args.legs == "all" else set(args.legs.split(","))
+        for name, fn in LEGS.items():
+            if name in wanted:
+                print(f"# leg {name}", file=sys.stderr, flush=True)
+                legs[name] = fn(args)
+        out["legs"] = legs
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(n, args.dt, args.cpu_seconds)
     print(json.dumps(out), flush=True)
+
+
+def _timed(step, steps: int, warmup: int) -> float:
+    """Seconds for `steps` calls of `step` after `warmup` untimed ones, device-synchronised."""
+    from xgrid_amd import _lib
+    lib = _lib.load()
+    for _ in range(warmup):
+        step()
+    _lib.check(lib.xg_synchronize())
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    _lib.check(lib.xg_synchronize())
+    return time.perf_counter() - t0
+
+
+def leg_diffusion_1d(args) -> dict:
+    """1-D diffusion, 4096 cells FP64, 500 calls: launch-bound (64 KiB of data)."""
+    import xgrid_amd
+    from xgrid_amd import workloads as W
+    xgrid_amd.init(precision="double", cacheroot=os.path.join(ROOT, ".xgrid"))
+    k = W.make_diffusion_1d()
+    u0, mask, dx = W.diffusion_1d_setup(4096)
+    nu = 0.3
+    dt = 0.2 * dx * dx / nu
+    u = xgrid_amd.Grid((4096,), float)
+    u.now[...] = u0
+    u.boundary[...] = mask
+    steps = 500
+    wall = _timed(lambda: k(u, nu, dt, dx), steps, 10)
+    return {"workload": "1-D diffusion nx=4096 FP64, 500 calls", "us_per_call": wall / steps * 1e6,
+            "cell_updates_per_s": 4096 * steps / wall, "counters": k.native.counters()}
+
+
+def leg_cavity_fp32(args) -> dict:
+    """The 2049^2 cavity with FP32 fields: throughput at the configured dt, and the
+    FP32-vs-FP64 error and FP64 Poisson residuals at a stable dt."""
+    import xgrid_amd
+    from xgrid_amd import workloads as W
+    from xgrid_amd.precision import CavityTwin
+    n = args.n
+    xgrid_amd.init(precision="float", cacheroot=os.path.join(ROOT, ".xgrid"))
+    k = W.make_cavity()
+    cfg = W.cavity_config(n, dt=args.dt)
+    masks = W.cavity_masks(n)
+    grids = [xgrid_amd.Grid((n, n), float) for _ in "bpuv"]
+    for g, c in zip(grids, "bpuv"):
+        g.boundary[...] = masks[c]
+    steps = min(args.steps, 300)
+    wall = _timed(lambda: k(*grids, cfg), steps, 5)
+    step_bytes = 1312 // 2 * (n - 2) ** 2            # FP64's 1312 B per interior cell, halved
+    out = {"workload": f"lid-driven cavity {n}x{n} FP32 fields, dt={args.dt}", "steps": steps,
+           "ms_per_step": wall / steps * 1e3, "cell_updates_per_s": n * n * steps / wall,
+           "step_roofline_frac": step_bytes / (wall / steps) / 1e9 / HBM_PEAK_GBS}
+    del grids
+    twin_steps = 50
+    twin = CavityTwin(n, dt=2e-6)
+    twin.run(twin_steps)
+    out["precision"] = dict(twin.report(), dt=2e-6,
+                            note="error = FP32 field - FP64 field over the interior, FP64 norms on the device; "
+                                 "residual = FP64 norm of lap(p) - b of each run")
+    return out
+
+
+def leg_heat_3d(args) -> dict:
+    """3-D heat, 512^3 FP32 storage, 1000 calls, then L2/Linf against the FP64 twin."""
+    from xgrid_amd.precision import Heat3DTwin
+    n, steps = 512, 1000
+    twin = Heat3DTwin(n, seed=0)
+    k32 = twin.kernels["float"]
+    g32 = twin.fields["float"]
+    wall = _timed(lambda: k32(g32, twin.nu, twin.dt, twin.dx), steps, 0)
+    twin.steps = steps
+    t64 = _timed(lambda: twin.kernels["double"](twin.fields["double"], twin.nu, twin.dt, twin.dx), steps, 0)
+    native = k32.native
+    sid = 0
+    ms = native.time_isolated(sid, 50)
+    kb = (n - 2) ** 3 * 4 * 2
+    return {"workload": f"3-D heat {n}^3 FP32 storage, 7-point FTCS, {steps} calls",
+            "ms_per_step": wall / steps * 1e3, "cell_updates_per_s": n ** 3 * steps / wall,
+            "fp64_ms_per_step": t64 / steps * 1e3,
+            "roofline": {"kernel": f"xg_k{sid} (interior 7-point update)", "bytes_per_launch": kb,
+                         "avg_launch_ms": ms, "achieved": kb / (ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+                         "frac": kb / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
+            "precision": twin.report()}
+
+
+LEGS = {"diffusion_1d": leg_diffusion_1d, "cavity_fp32": leg_cavity_fp32, "heat_3d_fp32": leg_heat_3d}
 
 
 def copy_probe(mib: int, iters: int = 20) -> None:

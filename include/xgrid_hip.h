@@ -57,6 +57,9 @@ typedef struct xg_stmt_desc {
   void* fn_main;                             /* hipFunction_t of the class kernel */
   void* fn_pingpong;                         /* hipFunction_t of the Jacobi sweep (PINGPONG) */
   void* fn_phase2;                           /* hipFunction_t storing a temporary (two-phase) */
+  void* fn_main_b;                           /* batched variants (many instances per launch) */
+  void* fn_pingpong_b;
+  void* fn_phase2_b;
 } xg_stmt_desc;
 
 /* A run of consecutive statements that may be chained in one workgroup. */
@@ -64,6 +67,7 @@ typedef struct xg_group_desc {
   int32_t first;
   int32_t count;
   void* fn;                                  /* hipFunction_t taking (uint32 enable, {hdr, refs, params} x count) */
+  void* fn_b;                                /* batched: one workgroup per instance */
 } xg_group_desc;
 
 /* A `for` loop whose body is stencil statements on one 2-D grid, which may run
@@ -79,6 +83,8 @@ typedef struct xg_loop_desc {
   int32_t threads;                           /* work-items per workgroup */
   void* fn;                                  /* hipFunction_t: tmax-bounded blocked iterations */
   void* fn_check;                            /* the same, proving every written core cell exact */
+  void* fn_b;                                /* batched variants */
+  void* fn_check_b;
 } xg_loop_desc;
 
 /* Function table handed to generated host stubs (the stub runs the kernel's
@@ -172,6 +178,57 @@ int xg_kernel_time_isolated(xg_kernel* k, int32_t sid, int32_t reps, double* ms_
 int xg_kernel_stmt_counts(xg_kernel* k, int64_t* counts, int32_t n, int32_t reset);
 /* The stub function table (pointers into this library). */
 const xg_api* xg_api_table(void);
+
+/* ---- batches: many instances of one kernel per launch ------------------ */
+/* No reference counterpart (the reference runs one kernel call at a time,
+ * xgrid/lang/operator.py:29-44).  A batch holds independent instances of one
+ * compiled kernel - each its own xg_kernel plan (xg_kernel_create with the
+ * same descriptors), its own grids of any size and its own arguments - and
+ * advances all of them one call per step, with one launch per statement
+ * covering the tiles of every instance, blocked loops batched the same way,
+ * and whole steps replayed as hipGraphs when they repeat.  Results equal those
+ * of calling each instance's kernel on its own. */
+typedef struct xg_batch xg_batch;
+/* packed_entry: the stub's xg_entry_packed(kernel, api, args); tick: the
+ * kernel rotates its grids' time levels on each call (Operator.tick). */
+int xg_batch_create(void* packed_entry, int32_t tick, xg_batch** out);
+int xg_batch_destroy(xg_batch* b);
+/* Add an instance: its plan, its arguments packed as the stub's struct
+ * xg_args (grid arguments are xg_grid handles), and its grid arguments in
+ * order (ticked once per occurrence before each call).  A grid may belong to
+ * one instance only.  Returns a stable id. */
+int xg_batch_add(xg_batch* b, xg_kernel* k, const void* args, int64_t arg_bytes, xg_grid* const* tick_grids,
+                 int32_t nticks, int64_t* id);
+int xg_batch_remove(xg_batch* b, int64_t id);
+int xg_batch_size(xg_batch* b);
+/* graphs: replay repeated steps; max_blocks: 0 launches one workgroup per
+ * tile, else at most this many (rounded down to a multiple of 8) walk the tiles. */
+int xg_batch_set_options(xg_batch* b, int32_t graphs, int64_t max_blocks);
+/* Advance every instance by `steps` calls (asynchronous, on the backend stream). */
+int xg_batch_step(xg_batch* b, int32_t steps);
+int xg_batch_counters(xg_batch* b, int64_t* launches, int64_t* replays, int64_t* captures, int64_t* steps,
+                      int64_t* validations);
+
+/* ---- FP64 reductions over one boundary class --------------------------- */
+/* No reference counterpart: the DSL has no reductions (a grid load is only
+ * allowed inside a stencil store, xgrid/lang/generator.py:62-65).  These give
+ * FP32-stored solvers an FP64 convergence test and error measure.  Element
+ * type codes say how to read each grid's bytes. */
+#define XG_T_F32 1
+#define XG_T_F64 2
+/* l2 = sqrt(sum d^2), linf = max |d| (NaN if any d is NaN) of
+ * d = a[level_a] - b[level_b] over the cells of class `klass` of a; b may be
+ * NULL (norms of a).  a and b have one shape.  cells = the class's cell count.
+ * Synchronises the backend stream. */
+int xg_grid_norm_diff(xg_grid* a, int32_t level_a, int32_t type_a, xg_grid* b, int32_t level_b, int32_t type_b,
+                      int32_t klass, double* l2, double* linf, int64_t* cells);
+/* Norms of the residual r = sum_a coef[a] (p[+e_a] - 2p + p[-e_a]) - rhs_scale * rhs
+ * of a Poisson equation over the cells of class `klass` of p, every cell of
+ * which must lie at least one cell inside the grid (else XG_ERR_ARG).  coef
+ * holds one coefficient per grid axis (axis 0 first).  Synchronises. */
+int xg_grid_poisson_residual(xg_grid* p, int32_t level_p, int32_t type_p, xg_grid* rhs, int32_t level_rhs,
+                             int32_t type_rhs, int32_t klass, const double* coef, double rhs_scale, double* l2,
+                             double* linf);
 
 /* ---- measurement probes ------------------------------------------------ */
 /* Device copy of `bytes` bytes repeated `iters` times; returns GB/s (read+write). */

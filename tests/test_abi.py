@@ -87,10 +87,34 @@ def test_generated_kernels_compile_for_gfx950(prec, request, tmp_path):
     from xgrid_amd import workloads as W
     from xgrid_amd.backend import lower_kernel
     from xgrid_amd.compilers import compile_device, compile_host
+    from xgrid_amd.backend import packed_args_type
+    from xgrid_amd.compilers import load_host
     for make in (W.make_diffusion_1d, W.make_cavity, W.make_heat_3d, W.make_convection_2d):
-        low, _ = lower_kernel(make())
+        op = make()
+        low, _ = lower_kernel(op)
         assert os.path.getsize(compile_device(low.device_src, str(tmp_path))) > 1000
-        assert os.path.exists(compile_host(low.stub_src, str(tmp_path), ["gcc"]))
+        stub = compile_host(low.stub_src, str(tmp_path), ["gcc"])
+        assert os.path.exists(stub)
+        # batches call the stub with every argument packed in one struct: ctypes must agree with C
+        lib = load_host(stub)
+        lib.xg_args_size.restype = ctypes.c_size_t
+        assert lib.xg_args_size() == ctypes.sizeof(packed_args_type(list(op.ir.args)))
+        assert hasattr(lib, "xg_entry_packed")
+
+
+def test_generated_device_source_has_batched_variants(double):
+    from xgrid_amd import workloads as W
+    from xgrid_amd.backend import lower_kernel
+    low, _ = lower_kernel(W.make_cavity())
+    src = low.device_src
+    for m in low.stmts:
+        assert f"{m.kernel}_b(" in src
+        if m.kernel_pp:
+            assert f"{m.kernel_pp}b(" in src
+    for g in low.groups:
+        assert f"{g.kernel}_b(" in src
+    for L in low.loops:
+        assert f"{L.kernel}_b(" in src and f"{L.kernel}_chkb(" in src
 
 
 def test_rtc_compile_error_is_reported(tmp_path):

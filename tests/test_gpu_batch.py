@@ -1,0 +1,166 @@
+"""Batched execution: every instance of a batch ends bit-exact with the same
+instance run alone through plain kernel calls (and with the oracle)."""
+import numpy as np
+import pytest
+
+from oracle import restate as R
+
+pytestmark = pytest.mark.gpu
+
+
+def _cavity(xg, n, dt=2e-6, nu=0.1):
+    from xgrid_amd import workloads as W
+    masks = W.cavity_masks(n)
+    grids = []
+    for c in "bpuv":
+        g = xg.Grid((n, n), float)
+        g.boundary[...] = masks[c]
+        grids.append(g)
+    return grids, W.cavity_config(n, dt=dt, nu=nu)
+
+
+def _assert_same(a, b):
+    for x, y in zip(a, b):
+        ax, by = x.now, y.now
+        assert ax.dtype == by.dtype and ax.shape == by.shape
+        if not np.array_equal(ax.view(np.uint8), by.view(np.uint8)):
+            d = np.argwhere(ax != by)
+            raise AssertionError(f"{len(d)} cells differ, first {d[:3].tolist()}")
+
+
+def _run_cavities(xg, sizes, steps, graphs=True, max_blocks=0, nus=None):
+    from xgrid_amd import workloads as W
+    from xgrid_amd.batch import Batch
+    kb, kp = W.make_cavity(), W.make_cavity()
+    batch = Batch(kb, graphs=graphs, max_blocks=max_blocks)
+    members = []
+    for i, n in enumerate(sizes):
+        nu = 0.1 if nus is None else nus[i]
+        gb, cfg = _cavity(xg, n, nu=nu)
+        ga, _ = _cavity(xg, n, nu=nu)
+        batch.add(*gb, cfg)
+        members.append((gb, ga, cfg))
+    batch.step(steps)
+    for gb, ga, cfg in members:
+        for _ in range(steps):
+            kp(*ga, cfg)
+    return batch, members
+
+
+@pytest.mark.parametrize("graphs,max_blocks", [(True, 0), (False, 0), (True, 64)])
+def test_batched_cavities_equal_alone(double, graphs, max_blocks):
+    sizes = [33, 64, 97, 130, 41, 200]
+    nus = [0.1, 0.05, 0.1, 0.02, 0.08, 0.1]
+    steps = 14
+    batch, members = _run_cavities(double, sizes, steps, graphs, max_blocks, nus)
+    for gb, ga, cfg in members:
+        _assert_same(gb, ga)
+    gb, ga, cfg = members[0]
+    ref = R.cavity(33, steps, cfg.rho, cfg.nu, cfg.dt, cfg.dx, cfg.dy)
+    for g, r in zip(gb, ref):
+        assert np.array_equal(g.now, r)
+    c = batch.counters()
+    assert c["steps"] == steps and c["validations"] >= 1
+    if graphs:
+        assert c["graph_captures"] >= 1 and c["graph_replays"] >= 1
+    else:
+        assert c["graph_replays"] == 0
+    batch.close()
+
+
+def test_batched_cavities_fp32_equal_alone(single):
+    batch, members = _run_cavities(single, [48, 80, 33], 9)
+    for gb, ga, cfg in members:
+        assert gb[1].now.dtype == np.float32
+        _assert_same(gb, ga)
+    batch.close()
+
+
+def test_batched_diffusion_2d_equal_alone(double):
+    from xgrid_amd import workloads as W
+    from xgrid_amd.batch import Batch
+    kb, kp = W.make_diffusion_2d(), W.make_diffusion_2d()
+    batch = Batch(kb)
+    rng = np.random.default_rng(4)
+    pairs = []
+    for n in (64, 100, 257, 31):
+        u0 = rng.random((n, n))
+        nu = float(rng.uniform(0.01, 0.1))
+        dx = 2.0 / (n - 1)
+        dt = 0.2 * dx * dx / nu
+        gs = []
+        for _ in range(2):
+            g = double.Grid((n, n), float)
+            g.now[...] = u0
+            g.boundary[...] = W.box_walls((n, n))
+            gs.append(g)
+        batch.add(gs[0], nu, dt, dx)
+        pairs.append((gs, (nu, dt, dx)))
+    batch.step(23)
+    for gs, a in pairs:
+        for _ in range(23):
+            kp(gs[1], *a)
+        _assert_same([gs[0]], [gs[1]])
+    batch.close()
+
+
+def test_batch_partitions_instances_by_control_flow(double):
+    """Instances whose calls take different paths (here: loop trip counts) are
+    walked in separate groups and still match their plain runs."""
+    from xgrid_amd import workloads as W
+    from xgrid_amd.batch import Batch
+    kb, kp = W.make_cavity(sweeps=7), W.make_cavity(sweeps=7)
+    batch = Batch(kb)
+    members = []
+    for n, nit in ((40, 3), (64, 12), (33, 3), (90, 0)):
+        gb, cfg = _cavity(double, n)
+        ga, _ = _cavity(double, n)
+        batch.add(*gb, cfg, nit)
+        members.append((gb, ga, cfg, nit))
+    batch.step(5)
+    for gb, ga, cfg, nit in members:
+        for _ in range(5):
+            kp(*ga, cfg, nit)
+        _assert_same(gb, ga)
+    batch.close()
+
+
+def test_batch_membership_changes_between_steps(double):
+    from xgrid_amd import workloads as W
+    from xgrid_amd.batch import Batch
+    kb, kp = W.make_cavity(), W.make_cavity()
+    batch = Batch(kb)
+    a, cfg_a = _cavity(double, 50)
+    b, cfg_b = _cavity(double, 70)
+    c, cfg_c = _cavity(double, 36)
+    ia = batch.add(*a, cfg_a)
+    batch.add(*b, cfg_b)
+    batch.step(4)
+    batch.remove(ia)
+    batch.add(*c, cfg_c)
+    batch.step(3)
+    assert len(batch) == 2
+    for grids, cfg, steps in ((a, cfg_a, 4), (b, cfg_b, 7), (c, cfg_c, 3)):
+        alone, _ = _cavity(double, grids[0].shape[0])
+        for _ in range(steps):
+            kp(*alone, cfg)
+        _assert_same(grids, alone)
+    with pytest.raises(Exception):
+        batch.add(*b, cfg_b)                 # a grid may belong to one instance only
+    batch.close()
+
+
+def test_batch_mixed_with_plain_calls_on_other_grids(double):
+    """A batch and plain calls of the same operator interleave without sharing state."""
+    from xgrid_amd import workloads as W
+    from xgrid_amd.batch import Batch
+    k = W.make_cavity()
+    batch = Batch(k)
+    gb, cfg = _cavity(double, 45)
+    gp, _ = _cavity(double, 45)
+    batch.add(*gb, cfg)
+    for _ in range(3):
+        batch.step(1)
+        k(*gp, cfg)
+    _assert_same(gb, gp)
+    batch.close()

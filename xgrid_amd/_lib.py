@@ -18,6 +18,7 @@ XG_MAX_HAZ = 8
 
 MODE_INPLACE, MODE_PINGPONG, MODE_TWOPHASE = 0, 1, 2
 DOM_BOX, DOM_LIST, DOM_MASKED, DOM_EMPTY = 0, 1, 2, 3
+T_F32, T_F64 = 1, 2
 
 
 class BackendError(RuntimeError):
@@ -32,11 +33,12 @@ class StmtDesc(Structure):
         ("store_off", c_int32 * 3), ("nhazard", c_int32), ("hazard", (c_int32 * 3) * XG_MAX_HAZ),
         ("param_bytes", c_int32), ("overstep", c_int32), ("group", c_int32), ("group_pos", c_int32),
         ("fn_main", c_void_p), ("fn_pingpong", c_void_p), ("fn_phase2", c_void_p),
+        ("fn_main_b", c_void_p), ("fn_pingpong_b", c_void_p), ("fn_phase2_b", c_void_p),
     ]
 
 
 class GroupDesc(Structure):
-    _fields_ = [("first", c_int32), ("count", c_int32), ("fn", c_void_p)]
+    _fields_ = [("first", c_int32), ("count", c_int32), ("fn", c_void_p), ("fn_b", c_void_p)]
 
 
 XG_LOOP_MAX = 8
@@ -45,7 +47,8 @@ XG_LOOP_MAX = 8
 class LoopDesc(Structure):
     _fields_ = [("nstmts", c_int32), ("sids", c_int32 * XG_LOOP_MAX), ("grid_arg", c_int32),
                 ("core0", c_int32), ("core1", c_int32), ("halo", c_int32), ("tmax", c_int32),
-                ("threads", c_int32), ("fn", c_void_p), ("fn_check", c_void_p)]
+                ("threads", c_int32), ("fn", c_void_p), ("fn_check", c_void_p), ("fn_b", c_void_p),
+                ("fn_check_b", c_void_p)]
 
 
 class Api(Structure):
@@ -91,6 +94,19 @@ _SIGS = {
     "xg_kernel_time_isolated": ([c_void_p, c_int32, c_int32, POINTER(c_double)], c_int),
     "xg_api_table": ([], c_void_p),
     "xg_copy_probe": ([c_int64, c_int32, POINTER(c_double), POINTER(c_double)], c_int),
+    "xg_batch_create": ([c_void_p, c_int32, POINTER(c_void_p)], c_int),
+    "xg_batch_destroy": ([c_void_p], c_int),
+    "xg_batch_add": ([c_void_p, c_void_p, c_void_p, c_int64, POINTER(c_void_p), c_int32, POINTER(c_int64)], c_int),
+    "xg_batch_remove": ([c_void_p, c_int64], c_int),
+    "xg_batch_size": ([c_void_p], c_int),
+    "xg_batch_set_options": ([c_void_p, c_int32, c_int64], c_int),
+    "xg_batch_step": ([c_void_p, c_int32], c_int),
+    "xg_batch_counters": ([c_void_p, POINTER(c_int64), POINTER(c_int64), POINTER(c_int64), POINTER(c_int64),
+                           POINTER(c_int64)], c_int),
+    "xg_grid_norm_diff": ([c_void_p, c_int32, c_int32, c_void_p, c_int32, c_int32, c_int32, POINTER(c_double),
+                           POINTER(c_double), POINTER(c_int64)], c_int),
+    "xg_grid_poisson_residual": ([c_void_p, c_int32, c_int32, c_void_p, c_int32, c_int32, c_int32, POINTER(c_double),
+                                  c_double, POINTER(c_double), POINTER(c_double)], c_int),
 }
 
 EXPORTED = tuple(_SIGS)

@@ -44,6 +44,12 @@ def _ctype_of(t):
     return t.ctype
 
 
+def packed_args_type(arg_types):
+    """ctypes mirror of the stub's ``struct xg_args`` (one field per kernel argument)."""
+    fields_ = [(f"a{i}", _ctype_of(t)) for i, (_, t) in enumerate(arg_types)] or [("xg_unused", ctypes.c_int32)]
+    return type("xg_args", (ctypes.Structure,), {"_fields_": fields_})
+
+
 class HipKernel:
     """A compiled kernel: code object, stub entry point and launch plan."""
 
@@ -105,11 +111,15 @@ class HipKernel:
             d.fn_main = fn(m.kernel)
             d.fn_pingpong = fn(m.kernel_pp)
             d.fn_phase2 = fn(m.phase2)
+            d.fn_main_b = fn(m.kernel + "_b")
+            d.fn_pingpong_b = fn(m.kernel_pp + "b") if m.kernel_pp else None
+            d.fn_phase2_b = fn(m.phase2 + "_b")
         groups = (_lib.GroupDesc * max(1, len(low.groups)))()
         for i, g in enumerate(low.groups):
             groups[i].first = g.first
             groups[i].count = g.count
             groups[i].fn = fn(g.kernel)
+            groups[i].fn_b = fn(g.kernel + "_b")
         loops = (_lib.LoopDesc * max(1, len(low.loops)))()
         for i, L in enumerate(low.loops):
             ld = loops[i]
@@ -123,10 +133,10 @@ class HipKernel:
             ld.threads = L.threads
             ld.fn = fn(L.kernel)
             ld.fn_check = fn(L.kernel + "_chk")
-        k = c_void_p()
-        _lib.check(self.lib.xg_kernel_create(len(low.stmts), descs, len(low.groups), groups, len(low.loops), loops,
-                                             len(low.grid_args), ctypes.byref(k)), "xg_kernel_create")
-        self.kernel = k.value
+            ld.fn_b = fn(L.kernel + "_b")
+            ld.fn_check_b = fn(L.kernel + "_chkb")
+        self._descs, self._groups, self._loops = descs, groups, loops
+        self.kernel = self.new_plan()
         self.set_options()
         self.api = self.lib.xg_api_table()
         self.stub = load_host(self.stub_path)
@@ -137,6 +147,20 @@ class HipKernel:
         self.ret = ret
         self.entry.restype = None if isinstance(ret, Void) else ret.ctype
         self.written = [grid_index[g] for g in info.written]
+        self.entry_packed = ctypes.cast(self.stub.xg_entry_packed, c_void_p).value
+        self.stub.xg_args_size.restype = ctypes.c_size_t
+        self.Args = packed_args_type(self.arg_types)
+        if ctypes.sizeof(self.Args) != self.stub.xg_args_size():
+            raise KernelError("packed argument layout differs between the stub and ctypes")
+
+    def new_plan(self) -> int:
+        """A fresh launch plan (xg_kernel) over this kernel's code: one per batch instance."""
+        low = self.lowered
+        k = c_void_p()
+        _lib.check(self.lib.xg_kernel_create(len(low.stmts), self._descs, len(low.groups), self._groups,
+                                             len(low.loops), self._loops, len(low.grid_args), ctypes.byref(k)),
+                   "xg_kernel_create")
+        return k.value
 
     def _param_size(self, m) -> int:
         fields_ = [(v.name, (v.type.element if isinstance(v.type, Pointer) else v.type).ctype) for v in m.params]
@@ -144,12 +168,13 @@ class HipKernel:
             fields_ = [("xg_unused", ctypes.c_int32)]
         return ctypes.sizeof(type("P", (ctypes.Structure,), {"_fields_": fields_}))
 
-    def set_options(self, graphs: bool | None = None, fuse: bool | None = None, blocked: bool | None = None):
+    def set_options(self, graphs: bool | None = None, fuse: bool | None = None, blocked: bool | None = None,
+                    plan: int | None = None):
         cfg = get_config()
         g = cfg.graphs and self.lowered.graphs_ok if graphs is None else graphs
         f = cfg.fuse if fuse is None else fuse
         b = cfg.blocked if blocked is None else blocked
-        _lib.check(self.lib.xg_kernel_set_options(self.kernel, int(g), int(f), int(b)))
+        _lib.check(self.lib.xg_kernel_set_options(self.kernel if plan is None else plan, int(g), int(f), int(b)))
 
     def loop_info(self, lid: int = 0) -> dict:
         n, it, st = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int32()
@@ -201,10 +226,11 @@ class HipKernel:
         _lib.check(self.lib.xg_kernel_stmt_time(self.kernel, sid, ctypes.byref(ms), ctypes.byref(n)))
         return ms.value, n.value
 
-    def __call__(self, *args):
+    def convert_args(self, args) -> tuple:
+        """Check arity and types (``xgrid/util/ffi.py:30-34``); returns (C values, Grid args in order)."""
         if len(args) != len(self.arg_types):
             raise TypeError(f"this function takes {len(self.arg_types)} argument ({len(args)} given)")
-        cargs = [self.kernel, self.api]
+        cargs = []
         grids = []
         for (name, t), a in zip(self.arg_types, args):
             if isinstance(t, GridType):
@@ -224,7 +250,11 @@ class HipKernel:
                     cargs.append(ctypes.pointer(a))
             else:
                 cargs.append(a)
-        result = self.entry(*cargs)
+        return cargs, grids
+
+    def __call__(self, *args):
+        cargs, grids = self.convert_args(args)
+        result = self.entry(self.kernel, self.api, *cargs)
         rc = self.lib.xg_kernel_status(self.kernel)
         for i in self.written:
             grids[i]._device_wrote()

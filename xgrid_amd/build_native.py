@@ -24,15 +24,23 @@ def hipcc() -> str:
     return os.path.join(ROCM, "bin", "hipcc")
 
 
+SOURCES = ["xgrid_runtime.cpp", "xgrid_reduce.cpp", "xgrid_batch.cpp"]
+
+
 def build_runtime(verbose: bool = False) -> str:
-    src = os.path.join(CSRC, "xgrid_runtime.cpp")
+    srcs = [os.path.join(CSRC, s) for s in SOURCES]
     tmp = LIB + ".tmp"
     cmd = [hipcc(), "-x", "hip", "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
-           "-ffp-contract=off", *NO_PACKED_FP32, f"-I{INCLUDE}", f"-I{CSRC}", src,
+           "-ffp-contract=off", *NO_PACKED_FP32, f"-I{INCLUDE}", f"-I{CSRC}", *srcs,
            "-o", tmp, f"-L{ROCM}/lib", "-lhiprtc", "-lamdhip64", f"-Wl,-rpath,{ROCM}/lib"]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
+    # every symbol of our own must resolve inside the library (a shared object links with holes)
+    nm = subprocess.run(["nm", "-D", "-C", "--undefined-only", tmp], capture_output=True, text=True, check=True)
+    own = [ln for ln in nm.stdout.splitlines() if "xgi::" in ln or " xg_" in ln]
+    if own:
+        raise RuntimeError("libxgrid_hip.so has unresolved symbols:\n" + "\n".join(own))
     os.replace(tmp, LIB)
     return LIB
 

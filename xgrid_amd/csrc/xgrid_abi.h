@@ -75,4 +75,24 @@ typedef struct xg_tb_hdr {
   int32_t uniform;                 /* every loaded view has the grid's shape */
 } xg_tb_hdr;                       /* 24 + 24 + 8 + 8 + 32 + 8 = 104 bytes */
 
+/* Batched launches: one launch runs one statement (or blocked loop, or chain
+ * of boundary statements) for many independent instances of one kernel.  A
+ * device table holds one entry per instance; instance i owns the global tiles
+ * [tile0, tile0 + ntiles) of the launch (entries sorted by tile0). */
+typedef struct xg_bent {
+  xg_launch_hdr hdr;         /* as for a single launch */
+  const void* refs;          /* this instance's xg_gview[] (in the table) */
+  const void* params;        /* its parameter block (in the table) */
+  int64_t tile0;
+  int64_t ntiles;
+} xg_bent;                   /* 120 + 8 + 8 + 8 + 8 = 152 bytes */
+
+typedef struct xg_btent {
+  xg_tb_hdr th;                          /* iters may differ between instances */
+  const void* refs[XG_TB_MAX_STMTS];
+  const void* params[XG_TB_MAX_STMTS];
+  int64_t tile0;
+  int64_t ntiles;
+} xg_btent;                              /* 104 + 64 + 64 + 16 = 248 bytes */
+
 #endif

@@ -159,9 +159,8 @@ struct TileCells {
   bool in[NC];
   int32_t tlo[3], thi[3];
 
-  __device__ __forceinline__ TileCells(const int32_t* lo, const int32_t* hi) {
+  __device__ __forceinline__ TileCells(const int32_t* lo, const int32_t* hi, uint32_t tile) {
     typedef Tile<D> TL;
-    const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
     const int tid = threadIdx.x;
     int t0 = (int)tile, t1 = 0, t2 = 0;
     if (D == 2) {
@@ -216,10 +215,10 @@ __device__ __forceinline__ bool tile_safe(const xg_launch_hdr& H, const int32_t*
 
 template <class S, int OV, bool MASKED>
 __device__ __forceinline__ void box_cells(const xg_launch_hdr& H, const typename S::Refs& R,
-                                          const typename S::Params& P) {
+                                          const typename S::Params& P, uint32_t tile) {
   constexpr int D = S::D;
   typedef typename S::T T;
-  TileCells<D> tc(H.dom.lo, H.dom.hi);
+  TileCells<D> tc(H.dom.lo, H.dom.hi, tile);
   constexpr int NC = TileCells<D>::NC;
   if (MASKED) {
 #pragma unroll
@@ -241,28 +240,32 @@ __device__ __forceinline__ void box_cells(const xg_launch_hdr& H, const typename
     if (tc.in[n]) store<S, OV>(H, tc.c[n][0], tc.c[n][1], tc.c[n][2], v[n]);
 }
 
+/* Tile `tile` of `ntiles` of the statement's class (a list class: a slice of
+ * the list, strided over the tiles). */
 template <class S, int OV>
-__device__ void run_main(const xg_launch_hdr& H, const typename S::Refs& R, const typename S::Params& P) {
+__device__ void run_main(const xg_launch_hdr& H, const typename S::Refs& R, const typename S::Params& P,
+                         uint32_t tile, int64_t ntiles) {
   constexpr int D = S::D;
   const xg_dom& dom = H.dom;
   if (dom.kind == XG_DOM_LIST) {
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < dom.count; k += stride) {
+    const int64_t stride = ntiles * blockDim.x;
+    for (int64_t k = (int64_t)tile * blockDim.x + threadIdx.x; k < dom.count; k += stride) {
       int i0, i1, i2;
       unlin<D>(H.out.shape, dom.list[k], i0, i1, i2);
       apply<S, OV>(H, R, P, i0, i1, i2);
     }
   } else if (dom.kind == XG_DOM_MASKED) {
-    box_cells<S, OV, true>(H, R, P);
+    box_cells<S, OV, true>(H, R, P, tile);
   } else {
-    box_cells<S, OV, false>(H, R, P);
+    box_cells<S, OV, false>(H, R, P, tile);
   }
 }
 
 /* Jacobi sweep: out = scratch, src = old level 0.  Every cell of the grid is
  * written exactly once, so after the sweep the two buffers swap roles. */
 template <class S, int OV>
-__device__ void run_pingpong(const xg_launch_hdr& H, const typename S::Refs& R, const typename S::Params& P) {
+__device__ void run_pingpong(const xg_launch_hdr& H, const typename S::Refs& R, const typename S::Params& P,
+                             uint32_t tile) {
   constexpr int D = S::D;
   typedef typename S::T T;
   const xg_dom& dom = H.dom;
@@ -270,7 +273,7 @@ __device__ void run_pingpong(const xg_launch_hdr& H, const typename S::Refs& R, 
   const int32_t* shape = H.out.shape;
   const T* __restrict__ src = static_cast<const T*>(H.src.ptr);
   T* __restrict__ out = static_cast<T*>(H.out.ptr);
-  TileCells<D> tc(zero, shape);
+  TileCells<D> tc(zero, shape, tile);
   constexpr int NC = TileCells<D>::NC;
   bool member[NC];
   int64_t lc[NC];
@@ -304,8 +307,8 @@ __device__ void run_pingpong(const xg_launch_hdr& H, const typename S::Refs& R, 
 
 /* Visit every cell of the box [lo, hi) handled by this workgroup's tile. */
 template <int D, class F>
-__device__ __forceinline__ void for_tile(const int32_t* lo, const int32_t* hi, F&& f) {
-  TileCells<D> tc(lo, hi);
+__device__ __forceinline__ void for_tile(const int32_t* lo, const int32_t* hi, uint32_t tile, F&& f) {
+  TileCells<D> tc(lo, hi, tile);
 #pragma unroll
   for (int n = 0; n < TileCells<D>::NC; ++n)
     if (tc.in[n]) f(tc.c[n][0], tc.c[n][1], tc.c[n][2]);
@@ -313,7 +316,7 @@ __device__ __forceinline__ void for_tile(const int32_t* lo, const int32_t* hi, F
 
 /* Phase 2 of a two-phase statement: store the temporary at the shifted cell. */
 template <class T, int D, int OV>
-__device__ void run_phase2(const xg_launch_hdr& H) {
+__device__ void run_phase2(const xg_launch_hdr& H, uint32_t tile, int64_t ntiles) {
   const xg_dom& dom = H.dom;
   const T* tmp = static_cast<const T*>(H.src.ptr);
   const int32_t* shape = H.out.shape;
@@ -321,18 +324,18 @@ __device__ void run_phase2(const xg_launch_hdr& H) {
     st<T, D, OV>(H.out, i0 + dom.so[0], i1 + dom.so[1], i2 + dom.so[2], tmp[lin<D>(shape, i0, i1, i2)]);
   };
   if (dom.kind == XG_DOM_LIST) {
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < dom.count; k += stride) {
+    const int64_t stride = ntiles * blockDim.x;
+    for (int64_t k = (int64_t)tile * blockDim.x + threadIdx.x; k < dom.count; k += stride) {
       int i0, i1, i2;
       unlin<D>(shape, dom.list[k], i0, i1, i2);
       f(i0, i1, i2);
     }
   } else if (dom.kind == XG_DOM_MASKED) {
-    for_tile<D>(dom.lo, dom.hi, [&](int i0, int i1, int i2) {
+    for_tile<D>(dom.lo, dom.hi, tile, [&](int i0, int i1, int i2) {
       if (dom.mask[lin<D>(shape, i0, i1, i2)] == dom.klass) f(i0, i1, i2);
     });
   } else {
-    for_tile<D>(dom.lo, dom.hi, f);
+    for_tile<D>(dom.lo, dom.hi, tile, f);
   }
 }
 
@@ -384,6 +387,32 @@ __device__ __forceinline__ T tb_ld(const T* lds, const uint8_t* vld, bool& ok, i
   const int i = r * RW + c;
   if (CHECK) ok = ok && vld[i];
   return lds[i];
+}
+
+/* ---- batched launches ---------------------------------------------------
+ * Global tile g of a batched launch belongs to the entry whose range
+ * [tile0, tile0 + ntiles) holds it (entries sorted by tile0).  Workgroups walk
+ * the global tiles with a stride of the grid size (the host keeps the grid a
+ * multiple of 8, so a workgroup stays on one XCD and the remap keeps each
+ * XCD's tiles contiguous). */
+template <class E>
+__device__ __forceinline__ int find_entry(const E* __restrict__ ents, int32_t n, int64_t g) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (ents[mid].tile0 <= g) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+template <class E, class F>
+__device__ __forceinline__ void for_batch_tiles(const E* __restrict__ ents, int32_t n, int64_t total, F&& f) {
+  for (int64_t p = blockIdx.x; p < total; p += gridDim.x) {
+    const int64_t g = xcd_remap((uint32_t)p, (uint32_t)total);
+    const E& e = ents[find_entry(ents, n, g)];
+    f(e, (uint32_t)(g - e.tile0));
+  }
 }
 
 /* Bitwise OR of `v` over the workgroup (all work-items must call it). */

@@ -17,76 +17,13 @@
 #include <utility>
 #include <vector>
 
-#include "xgrid_hip.h"
-#include "xgrid_abi.h"
+#include "xgrid_internal.h"
+#include "xgrid_kernel.h"
 
-namespace {
+namespace xgi {
 
 thread_local std::string t_err;
-
-int fail(int code, const std::string& msg) {
-  t_err = msg;
-  return code;
-}
-
-#define XG_HIP(call)                                                                      \
-  do {                                                                                    \
-    hipError_t e_ = (call);                                                               \
-    if (e_ != hipSuccess)                                                                 \
-      return fail(XG_ERR_HIP, std::string(#call) + " failed: " + hipGetErrorString(e_)); \
-  } while (0)
-
-struct Device {
-  int id = -1;
-  hipStream_t stream = nullptr;
-  bool ready = false;
-  int cus = 0;
-  int64_t hbm = 0;
-  char name[256] = {0};
-};
 Device g_dev;
-uint64_t g_mask_version = 1;
-
-constexpr int kThreads = 256;
-constexpr int kGroupThreads = 1024;
-constexpr int64_t kFuseMaxCells = 1 << 16;  // a chained statement walks at most this many cells
-
-inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
-
-}  // namespace
-
-// ---------------------------------------------------------------------------
-// grids
-// ---------------------------------------------------------------------------
-struct ClassPlan {
-  int32_t kind = XG_DOM_EMPTY;
-  int64_t count = 0;
-  int32_t lo[3] = {0, 0, 0};
-  int32_t hi[3] = {1, 1, 1};
-  int32_t* d_list = nullptr;
-};
-
-struct xg_grid {
-  int32_t dims = 1;
-  int32_t shape[3] = {1, 1, 1};
-  int64_t n = 0;
-  int32_t elem = 0;
-  std::vector<void*> levels;
-  void* scratch = nullptr;
-  std::vector<int32_t> h_mask;  // empty: every cell is class 0
-  int32_t* d_mask = nullptr;
-  uint64_t mask_version = 0;
-  std::unordered_map<int32_t, ClassPlan> plans;
-
-  size_t bytes() const { return (size_t)n * (size_t)elem; }
-  const ClassPlan& plan(int32_t k) const {
-    static const ClassPlan empty;
-    auto it = plans.find(k);
-    return it == plans.end() ? empty : it->second;
-  }
-};
-
-namespace {
 
 int alloc_zero(size_t bytes, void** out) {
   void* p = nullptr;
@@ -97,6 +34,14 @@ int alloc_zero(size_t bytes, void** out) {
   *out = p;
   return XG_OK;
 }
+
+}  // namespace xgi
+
+using namespace xgi;
+
+namespace {
+
+uint64_t g_mask_version = 1;
 
 void free_plans(xg_grid* g) {
   for (auto& kv : g->plans)
@@ -117,86 +62,12 @@ void default_plan(xg_grid* g) {
   g->mask_version = g_mask_version++;
 }
 
-int check_ready() {
-  if (!g_dev.ready) return fail(XG_ERR_STATE, "xg_init has not been called");
-  return XG_OK;
-}
-
 }  // namespace
 
 // ---------------------------------------------------------------------------
 // kernels
 // ---------------------------------------------------------------------------
-namespace {
-
-enum CallMode { kDirect = 0, kCapture = 1, kReplay = 2 };
-
-struct StmtPlan {
-  xg_stmt_desc d;
-  uint64_t haz_version = 0;
-  bool haz = false;
-};
-
-struct Pending {
-  int32_t sid;
-  size_t off;   // offset of its params in the arena
-  int64_t bytes;
-};
-
-struct GraphEntry {
-  std::vector<uint8_t> key;
-  uint64_t hash = 0;
-  int seen = 0;
-  hipGraphExec_t exec = nullptr;
-  std::vector<std::vector<void*>> post_levels;
-  std::vector<void*> post_scratch;
-  uint64_t last_use = 0;
-};
-
-struct EventPair {
-  hipEvent_t a, b;
-};
-
-struct LoopPlan {
-  xg_loop_desc d;
-  const void* grid = nullptr;   // the grid the plan was validated on
-  uint64_t mask_version = 0;    // ... and its mask
-  int state = 0;                // 1 exact, 0 unknown, -1 declined
-  int64_t launches = 0, iterations = 0;
-  std::vector<std::vector<uint8_t>> last_params;
-};
-
-}  // namespace
-
-struct xg_kernel {
-  std::vector<StmtPlan> stmts;
-  std::vector<xg_group_desc> groups;
-  int32_t ngrids = 0;
-  std::vector<xg_grid*> grids;
-  int status = XG_OK;
-  std::string status_msg;
-  int mode = kDirect;
-  GraphEntry* cur = nullptr;
-  std::vector<Pending> pending;
-  std::vector<uint8_t> arena;
-  int32_t pending_group = -1;
-  bool graphs = true, fuse = true;
-  bool graphs_auto_off = false;
-  int64_t graph_misses = 0, graph_hits = 0;
-  std::vector<GraphEntry> cache;
-  uint64_t tick = 0;
-  int64_t n_launch = 0, n_fused = 0, n_replay = 0, n_capture = 0;
-  int32_t timed_sid = -1;
-  std::vector<EventPair> events;
-  std::vector<LoopPlan> loops;
-  bool blocked = true;
-  int32_t* d_fail = nullptr;        // validation flag of blocked launches
-  std::vector<int64_t> exec_count;  // stmt() calls per statement (all modes)
-  std::vector<std::vector<uint8_t>> last_params;  // parameters of each statement's last call
-  std::vector<uint8_t> key_buf;
-};
-
-namespace {
+namespace xgi {
 
 uint64_t fnv1a(const uint8_t* p, size_t n) {
   uint64_t h = 1469598103934665603ull;
@@ -486,6 +357,50 @@ int flush(xg_kernel* k) {
   return rc;
 }
 
+int ensure_scratch(xg_kernel* k) {
+  // scratch for every grid a statement stores to (Jacobi, two-phase, hazards)
+  for (auto& sp : k->stmts) {
+    xg_grid* G = k->grids[(size_t)sp.d.store_arg];
+    if (!G->scratch) {
+      int rc = alloc_zero(G->bytes(), &G->scratch);
+      if (rc) return rc;
+    }
+  }
+  return XG_OK;
+}
+
+bool loop_blockable(xg_kernel* k, LoopPlan& L) {
+  xg_grid* G = k->grids[(size_t)L.d.grid_arg];
+  // another argument bound to the loop's grid would miss the LDS copy
+  for (int j = 0; j < L.d.nstmts; ++j) {
+    const xg_stmt_desc& d = k->stmts[(size_t)L.d.sids[j]].d;
+    for (int r = 0; r < d.nrefs; ++r)
+      if (d.ref_arg[r] != L.d.grid_arg && k->grids[(size_t)d.ref_arg[r]] == G) return false;
+  }
+  if (L.grid != (const void*)G || L.mask_version != G->mask_version) {
+    L.state = 0;
+    L.grid = G;
+    L.mask_version = G->mask_version;
+  }
+  return true;
+}
+
+void fill_tb_hdr(xg_kernel* k, LoopPlan& L, xg_tb_hdr& TH, int iters, int32_t* fail_flag) {
+  xg_grid* G = k->grids[(size_t)L.d.grid_arg];
+  std::memset(&TH, 0, sizeof TH);
+  fill_view(TH.src, G, G->levels[0]);
+  fill_view(TH.dst, G, G->scratch);
+  TH.mask = G->h_mask.empty() ? nullptr : G->d_mask;
+  TH.fail = fail_flag;
+  TH.iters = iters;
+  TH.uniform = 1;
+  for (int j = 0; j < L.d.nstmts; ++j) {
+    const xg_stmt_desc& d = k->stmts[(size_t)L.d.sids[j]].d;
+    TH.klass[j] = d.klass;
+    if (!uniform_refs(k, d)) TH.uniform = 0;
+  }
+}
+
 void destroy_entry(GraphEntry& e) {
   if (e.exec) (void)hipGraphExecDestroy(e.exec);
   e.exec = nullptr;
@@ -511,13 +426,12 @@ int api_begin(xg_kernel* k, xg_grid* const* grids, int32_t ngrids, const void* k
   k->grids.assign(grids, grids + ngrids);
   for (auto* G : k->grids)
     if (!G) return set_status(k, fail(XG_ERR_ARG, "null grid handle"));
-  // scratch for every grid a statement stores to (Jacobi, two-phase, hazards)
-  for (auto& sp : k->stmts) {
-    xg_grid* G = k->grids[(size_t)sp.d.store_arg];
-    if (!G->scratch) {
-      int rc = alloc_zero(G->bytes(), &G->scratch);
-      if (rc) return set_status(k, rc);
-    }
+  if (int rc = ensure_scratch(k)) return set_status(k, rc);
+  if (k->record_next) {
+    k->mode = kRecord;
+    k->rec.clear();
+    k->rec_arena.clear();
+    return XG_OK;
   }
   if (!k->graphs || k->graphs_auto_off) return XG_OK;
   // key: scalar arguments + state of every bound grid
@@ -588,6 +502,14 @@ int api_stmt(xg_kernel* k, int32_t sid, const void* params, int64_t bytes) {
   }
   const xg_stmt_desc& d = k->stmts[(size_t)sid].d;
   if (bytes != d.param_bytes) return set_status(k, fail(XG_ERR_ARG, "parameter block size mismatch"));
+  if (k->mode == kRecord) {
+    k->exec_count[(size_t)sid]++;
+    Rec r{0, sid, 0, k->rec_arena.size()};
+    k->rec_arena.resize(r.off + (size_t)((bytes + 7) & ~7ll), 0);
+    std::memcpy(k->rec_arena.data() + r.off, params, (size_t)bytes);
+    k->rec.push_back(r);
+    return XG_OK;
+  }
   k->exec_count[(size_t)sid]++;
   const int32_t gi = d.group;
   const bool extends = !k->pending.empty() && gi >= 0 && gi == k->pending_group &&
@@ -614,6 +536,11 @@ int api_stmt(xg_kernel* k, int32_t sid, const void* params, int64_t bytes) {
 
 int api_end(xg_kernel* k) {
   int rc = XG_OK;
+  if (k->mode == kRecord) {
+    k->mode = kDirect;
+    k->record_next = false;
+    return k->status;
+  }
   if (k->mode != kReplay && k->status == XG_OK) {
     rc = flush(k);
     if (rc) set_status(k, rc);
@@ -664,23 +591,14 @@ const int32_t* api_shape(xg_grid* g) { return g ? g->shape : nullptr; }
 // One launch of a blocked loop: `iters` iterations from level 0 into the scratch.
 int launch_blocked(xg_kernel* k, LoopPlan& L, xg_grid* G, const void* const* params, int iters, bool check) {
   xg_tb_hdr TH;
-  std::memset(&TH, 0, sizeof TH);
-  fill_view(TH.src, G, G->levels[0]);
-  fill_view(TH.dst, G, G->scratch);
-  TH.mask = G->h_mask.empty() ? nullptr : G->d_mask;
-  TH.fail = k->d_fail;
-  TH.iters = iters;
-  TH.uniform = 1;
+  fill_tb_hdr(k, L, TH, iters, k->d_fail);
   const int m = L.d.nstmts;
   std::vector<std::vector<xg_gview>> refs((size_t)m);
   std::vector<void*> args;
   args.reserve(1 + 2 * (size_t)m);
   args.push_back(&TH);
   for (int j = 0; j < m; ++j) {
-    const xg_stmt_desc& d = k->stmts[(size_t)L.d.sids[j]].d;
-    TH.klass[j] = d.klass;
-    if (!uniform_refs(k, d)) TH.uniform = 0;
-    int rc = fill_refs(k, d, refs[(size_t)j]);
+    int rc = fill_refs(k, k->stmts[(size_t)L.d.sids[j]].d, refs[(size_t)j]);
     if (rc) return rc;
   }
   for (int j = 0; j < m; ++j) {
@@ -699,6 +617,19 @@ int api_loop(xg_kernel* k, int32_t lid, int64_t trip, const void* const* params,
     return 1;
   }
   LoopPlan& L = k->loops[(size_t)lid];
+  if (k->mode == kRecord) {
+    Rec r{1, lid, trip, k->rec_arena.size()};
+    for (int j = 0; j < n; ++j) {
+      const int32_t sid = L.d.sids[j];
+      const int64_t b = k->stmts[(size_t)sid].d.param_bytes;
+      const size_t at = k->rec_arena.size();
+      k->rec_arena.resize(at + (size_t)((b + 7) & ~7ll), 0);
+      std::memcpy(k->rec_arena.data() + at, params[j], (size_t)b);
+      k->exec_count[(size_t)sid] += trip;
+    }
+    k->rec.push_back(r);
+    return 1;
+  }
   auto account = [&]() {
     for (int j = 0; j < n; ++j) {
       const int32_t sid = L.d.sids[j];
@@ -718,17 +649,7 @@ int api_loop(xg_kernel* k, int32_t lid, int64_t trip, const void* const* params,
   }
   if (!k->blocked || trip <= 0 || !L.d.fn) return 0;
   xg_grid* G = k->grids[(size_t)L.d.grid_arg];
-  // another argument bound to the loop's grid would miss the LDS copy
-  for (int j = 0; j < n; ++j) {
-    const xg_stmt_desc& d = k->stmts[(size_t)L.d.sids[j]].d;
-    for (int r = 0; r < d.nrefs; ++r)
-      if (d.ref_arg[r] != L.d.grid_arg && k->grids[(size_t)d.ref_arg[r]] == G) return 0;
-  }
-  if (L.grid != (const void*)G || L.mask_version != G->mask_version) {
-    L.state = 0;
-    L.grid = G;
-    L.mask_version = G->mask_version;
-  }
+  if (!loop_blockable(k, L)) return 0;
   if (L.state < 0) return 0;
   if (L.state == 0 && k->mode == kCapture) return 0;  // validation needs a synchronisation
   int rc = flush(k);
@@ -779,7 +700,6 @@ int api_loop(xg_kernel* k, int32_t lid, int64_t trip, const void* const* params,
   return 1;
 }
 
-const xg_api g_api = {api_begin, api_stmt, api_end, api_shape, api_loop};
 
 // copy probe kernel (calibration of HBM bandwidth and of byte counters)
 __global__ void __launch_bounds__(256) xg_copy_kernel(const double4* __restrict__ in, double4* __restrict__ out, int64_t n) {
@@ -788,6 +708,12 @@ __global__ void __launch_bounds__(256) xg_copy_kernel(const double4* __restrict_
 }
 
 }  // namespace
+
+#if !defined(__HIP_DEVICE_COMPILE__)   // host function pointers: no device copy
+namespace xgi {
+const xg_api g_api = {api_begin, api_stmt, api_end, api_shape, api_loop};
+}  // namespace xgi
+#endif
 
 // ---------------------------------------------------------------------------
 // C ABI

@@ -367,11 +367,21 @@ class Emitter:
             self._prefetched.add(sid)
         out.append("};")
         args = f"const xg_launch_hdr H, const xg_r{sid} R, const xg_p{sid} P"
+        remap = "xg::xcd_remap(blockIdx.x, gridDim.x)"
         out.append(f'extern "C" __global__ void __launch_bounds__(256) xg_k{sid}({args}) '
-                   f"{{ xg::run_main<xg_s{sid}, XG_OVERSTEP>(H, R, P); }}")
+                   f"{{ xg::run_main<xg_s{sid}, XG_OVERSTEP>(H, R, P, {remap}, gridDim.x); }}")
+        # batched: one launch over the tiles of many instances (entries in a device table)
+        bargs = "const xg_bent* __restrict__ E, const int32_t n, const int64_t total"
+        cast = f"*static_cast<const xg_r{sid}*>(e.refs), *static_cast<const xg_p{sid}*>(e.params)"
+        out.append(f'extern "C" __global__ void __launch_bounds__(256) xg_k{sid}_b({bargs}) '
+                   f"{{ xg::for_batch_tiles(E, n, total, [&](const xg_bent& e, uint32_t t) "
+                   f"{{ xg::run_main<xg_s{sid}, XG_OVERSTEP>(e.hdr, {cast}, t, e.ntiles); }}); }}")
         if st.mode == "pingpong":
             out.append(f'extern "C" __global__ void __launch_bounds__(256) xg_k{sid}_pp({args}) '
-                       f"{{ xg::run_pingpong<xg_s{sid}, XG_OVERSTEP>(H, R, P); }}")
+                       f"{{ xg::run_pingpong<xg_s{sid}, XG_OVERSTEP>(H, R, P, {remap}); }}")
+            out.append(f'extern "C" __global__ void __launch_bounds__(256) xg_k{sid}_ppb({bargs}) '
+                       f"{{ xg::for_batch_tiles(E, n, total, [&](const xg_bent& e, uint32_t t) "
+                       f"{{ xg::run_pingpong<xg_s{sid}, XG_OVERSTEP>(e.hdr, {cast}, t); }}); }}")
 
     def _device_function(self, op, out: list):
         f = op.ir
@@ -405,7 +415,7 @@ class Emitter:
                         rh[d] = max(rh[d], n.offsets[d])
         o = out.append
         o(f"template <bool CHECK>")
-        o(f"__device__ __forceinline__ void {L.kernel}_body({', '.join(params)}) {{")
+        o(f"__device__ __forceinline__ void {L.kernel}_body({', '.join(params)}, const int tile) {{")
         o(f"  typedef {T} T;")
         o(f"  constexpr int TBT = {L.threads};")
         o(f"  constexpr int C0 = {c0}, C1 = {c1}, HL = {L.halo}, RHH = C0 + 2 * HL, RWW = C1 + 2 * HL;")
@@ -426,7 +436,6 @@ class Emitter:
         o("  const int32_t* sh = TH.src.shape;")
         o("  const int N0 = sh[0], N1 = sh[1];")
         o("  const int nt1 = (N1 + C1 - 1) / C1;")
-        o("  const int tile = (int)xg::xcd_remap(blockIdx.x, gridDim.x);")
         o("  const int g0 = (tile / nt1) * C0, g1 = (tile % nt1) * C1;")
         o("  const int r0 = g0 - HL, r1 = g1 - HL;")
         o("  const T* __restrict__ src = static_cast<const T*>(TH.src.ptr);")
@@ -572,11 +581,20 @@ class Emitter:
         o("}")
         kargs = ["const xg_tb_hdr TH"] + [f"const xg_r{s.stmt.sid} R{j}, const xg_p{s.stmt.sid} P{j}"
                                           for j, s in enumerate(L.stmts)]
-        call = ", ".join(["TH"] + [f"R{j}, P{j}" for j in range(m)])
+        call = ", ".join(["TH"] + [f"R{j}, P{j}" for j in range(m)] + ["(int)xg::xcd_remap(blockIdx.x, gridDim.x)"])
         o(f'extern "C" __global__ void __launch_bounds__({L.threads}) {L.kernel}({", ".join(kargs)}) '
           f"{{ {L.kernel}_body<false>({call}); }}")
         o(f'extern "C" __global__ void __launch_bounds__({L.threads}) {L.kernel}_chk({", ".join(kargs)}) '
           f"{{ {L.kernel}_body<true>({call}); }}")
+        # batched: the tiles of many instances; the barrier lets a workgroup reuse its LDS for the next tile
+        bcall = ", ".join(["e.th"] + [f"*static_cast<const xg_r{s.stmt.sid}*>(e.refs[{j}]), "
+                                      f"*static_cast<const xg_p{s.stmt.sid}*>(e.params[{j}])"
+                                      for j, s in enumerate(L.stmts)] + ["(int)t"])
+        for suffix, chk in (("_b", "false"), ("_chkb", "true")):
+            o(f'extern "C" __global__ void __launch_bounds__({L.threads}) {L.kernel}{suffix}('
+              f"const xg_btent* __restrict__ E, const int32_t n, const int64_t total) "
+              f"{{ xg::for_batch_tiles(E, n, total, [&](const xg_btent& e, uint32_t t) "
+              f"{{ {L.kernel}_body<{chk}>({bcall}); __syncthreads(); }}); }}")
 
     def device_source(self, stmt_meta: list, groups: list, loops: list = ()) -> str:
         out = ["// generated by xgrid_amd for the MI355X backend", '#include "xgrid_device.h"',
@@ -603,8 +621,13 @@ class Emitter:
             if key in done:
                 continue
             done.add(key)
+            ph2 = f"xg::run_phase2<{T}, {m.stmt.dimension}, XG_OVERSTEP>"
             out.append(f'extern "C" __global__ void __launch_bounds__(256) {m.phase2}(const xg_launch_hdr H) '
-                       f"{{ xg::run_phase2<{T}, {m.stmt.dimension}, XG_OVERSTEP>(H); }}")
+                       f"{{ {ph2}(H, xg::xcd_remap(blockIdx.x, gridDim.x), gridDim.x); }}")
+            out.append(f'extern "C" __global__ void __launch_bounds__(256) {m.phase2}_b('
+                       f"const xg_bent* __restrict__ E, const int32_t n, const int64_t total) "
+                       f"{{ xg::for_batch_tiles(E, n, total, [&](const xg_bent& e, uint32_t t) "
+                       f"{{ {ph2}(e.hdr, t, e.ntiles); }}); }}")
         for g in groups:
             params = ["const uint32_t en"]
             body = []
@@ -616,6 +639,19 @@ class Emitter:
                     body.append(f"  if (en & {1 << pos}u) {{ xg::run_block<xg_s{sid}, XG_OVERSTEP>(H{pos}, R{pos}, P{pos}); __syncthreads(); }}")
             out.append(f'extern "C" __global__ void __launch_bounds__(1024) {g.kernel}({", ".join(params)}) {{')
             out.extend(body)
+            out.append("}")
+            # batched: workgroup i chains the statements of instance i (entries E[i * count + pos];
+            # an empty domain skips a statement)
+            out.append(f'extern "C" __global__ void __launch_bounds__(1024) {g.kernel}_b('
+                       f"const xg_bent* __restrict__ E, const int32_t count) {{")
+            out.append("  const xg_bent* I = E + (int64_t)blockIdx.x * count;")
+            for pos in range(g.count):
+                m = stmt_meta[g.first + pos]
+                sid = m.stmt.sid
+                if m.stmt.mode == "inplace":
+                    out.append(f"  if (I[{pos}].hdr.dom.kind != XG_DOM_EMPTY) {{ xg::run_block<xg_s{sid}, XG_OVERSTEP>("
+                               f"I[{pos}].hdr, *static_cast<const xg_r{sid}*>(I[{pos}].refs), "
+                               f"*static_cast<const xg_p{sid}*>(I[{pos}].params)); __syncthreads(); }}")
             out.append("}")
         return "\n".join(out) + "\n"
 
@@ -784,6 +820,17 @@ class Emitter:
             out.append("  xg_A->end(xg_K);")
             out.append("  return xg_r;")
         out.append("}")
+        # the same entry with every argument in one struct (laid out like a ctypes
+        # Structure of the same fields): batches call it from C without Python
+        fields_ = " ".join(f"{param_decl(f'a{i}', t)};" for i, (n, t) in enumerate(f.args)) or "int32_t xg_unused;"
+        out.append(f"struct xg_args {{ {fields_} }};")
+        unpack = ", ".join(["xg_K", "xg_A"] + [f"xg_a->a{i}" for i in range(len(f.args))])
+        out.append("void xg_entry_packed(void* xg_K, const xg_api_t* xg_A, const void* xg_p) {")
+        out.append("  const struct xg_args* xg_a = (const struct xg_args*)xg_p;")
+        out.append("  (void)xg_a;")
+        out.append(f"  {self.entry}({unpack});")
+        out.append("}")
+        out.append("size_t xg_args_size(void) { return sizeof(struct xg_args); }")
         return "\n".join(out) + "\n"
 
     def _host_expr(self):
