@@ -46,6 +46,14 @@ def main():
                     help="only run the device copy kernel on MIB MiB (byte-counter calibration)")
     ap.add_argument("--legs", default="all",
                     help="secondary workloads after the headline one: all, none, or a comma list")
+    ap.add_argument("--batch-steps", type=int, default=100, help="steps of the batching leg")
+    ap.add_argument("--batch-max-blocks", type=int, default=0,
+                    help="persistent grid cap of batched launches (0: one workgroup per tile)")
+    ap.add_argument("--serve-requests", type=int, default=48, help="requests of the serving trace (400 = full)")
+    ap.add_argument("--serve-budget-gib", type=float, default=16.0)
+    ap.add_argument("--serve-tick", type=int, default=8, help="kernel calls per serving tick")
+    ap.add_argument("--serve-low-rate", type=float, default=2.0, help="arrival rate of the below-saturation run")
+    ap.add_argument("--serve-low-requests", type=int, default=12)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this backend runs on one GPU; multi-GPU runs are not part of this build"}))
@@ -258,7 +266,94 @@ def leg_heat_3d(args) -> dict:
             "precision": twin.report()}
 
 
-LEGS = {"diffusion_1d": leg_diffusion_1d, "cavity_fp32": leg_cavity_fp32, "heat_3d_fp32": leg_heat_3d}
+def leg_batching(args) -> dict:
+    """256 independent cavity / 2-D diffusion instances (side log-uniform in [64, 1024],
+    seed 1): K steps of all of them batched against the same K steps run instance by
+    instance with plain kernel calls; results must be identical."""
+    import numpy as np
+    from xgrid_amd import workloads as W
+    from xgrid_amd.multi import MultiSolver
+    from xgrid_amd import _lib
+    lib = _lib.load()
+    specs = W.batch_specs(256, seed=1)
+    K = args.batch_steps
+    warm = 2
+    cells = sum(s.n * s.n for s in specs)
+    out = {"workload": f"{len(specs)} instances ({sum(s.kind == 'cavity' for s in specs)} cavity, "
+                       f"{sum(s.kind == 'diffusion' for s in specs)} 2-D diffusion), side log-uniform [64, 1024], "
+                       f"seed 1, {K} steps", "cells": cells}
+    for prec in ("double", "float"):
+        ms = MultiSolver(prec, max_blocks=args.batch_max_blocks)
+        handles = [ms.add(s) for s in specs]
+        ms.step(warm)
+        _lib.check(lib.xg_synchronize())
+        t0 = time.perf_counter()
+        ms.step(K)
+        _lib.check(lib.xg_synchronize())
+        t_batch = time.perf_counter() - t0
+        plain_before = {k: op.native.counters() for k, op in ms.ops.items()}
+        plain = []
+        for s in specs:
+            grids, scalars = ms.grids_for(s)
+            for _ in range(warm):
+                ms.plain_call(s, grids, scalars)
+            plain.append((grids, scalars))
+        _lib.check(lib.xg_synchronize())
+        t0 = time.perf_counter()
+        for s, (grids, scalars) in zip(specs, plain):
+            for _ in range(K):
+                ms.plain_call(s, grids, scalars)
+        _lib.check(lib.xg_synchronize())
+        t_plain = time.perf_counter() - t0
+        same = all(np.array_equal(a.now, b.now) for h, (grids, _) in zip(handles, plain)
+                   for a, b in zip(ms.grids(h), grids))
+        out["f64" if prec == "double" else "f32"] = {
+            "batched_s": t_batch, "plain_s": t_plain, "speedup": t_plain / t_batch,
+            "solves_per_s_batched": len(specs) / t_batch, "solves_per_s_plain": len(specs) / t_plain,
+            "cell_updates_per_s_batched": cells * K / t_batch, "ms_per_batched_step": t_batch / K * 1e3,
+            "identical_to_plain": bool(same), "counters": ms.counters(),
+            "plain_counters": {k: {c: v - plain_before[k][c] for c, v in op.native.counters().items()}
+                               for k, op in ms.ops.items()}}
+        ms.close()
+        del plain
+    out["note"] = "a solve = K steps of one instance; plain = each instance's kernel called K times, one instance after another"
+    return out
+
+
+def leg_serving(args) -> dict:
+    """Poisson arrivals (200/s, seed 2) of FP32 cavity solves, side log-uniform in
+    [128, 2048], 100-2000 steps, served with continuous batching under an HBM budget;
+    the same trace served one request at a time with plain calls; and a lower
+    arrival rate below saturation."""
+    import numpy as np
+    from xgrid_amd import workloads as W
+    from xgrid_amd.serving import Server, serve_plain
+    budget = int(args.serve_budget_gib * (1 << 30))
+    trace = W.serving_specs(args.serve_requests, seed=2, rate=200.0)
+    srv = Server("float", hbm_budget=budget, steps_per_tick=args.serve_tick)
+    got, summ = srv.run(trace)
+    summ.update(peak_resident_requests=srv.peak_active, peak_resident_bytes=srv.peak_used, ticks=srv.ticks)
+    ref, summ_plain = serve_plain(trace, "float", solver=srv.ms)
+    same = all(np.array_equal(got[r].fields[c], ref[r].fields[c]) for r in got for c in "bpuv")
+    del got, ref
+    low = W.serving_specs(args.serve_low_requests, seed=3, rate=args.serve_low_rate)
+    got_low, summ_low = srv.run(low)
+    summ_low.update(peak_resident_requests=srv.peak_active)
+    del got_low
+    srv.close()
+    work = sum(s.n * s.n * s.steps for s in trace)
+    return {"workload": f"{len(trace)} requests of the seed-2 trace (Poisson 200/s, cavity FP32, side log-uniform "
+                        f"[128, 2048], 100-2000 steps, nu in [0.001, 0.1]), HBM budget {args.serve_budget_gib} GiB, "
+                        f"{args.serve_tick} steps per tick",
+            "cell_steps": work, "batched": summ, "plain_fifo": summ_plain,
+            "speedup_solves_per_s": summ["solves_per_s"] / summ_plain["solves_per_s"],
+            "identical_to_plain": bool(same),
+            "below_saturation": dict(summ_low, rate_per_s=args.serve_low_rate,
+                                     workload=f"{len(low)} requests, seed 3, Poisson {args.serve_low_rate}/s")}
+
+
+LEGS = {"diffusion_1d": leg_diffusion_1d, "cavity_fp32": leg_cavity_fp32, "heat_3d_fp32": leg_heat_3d,
+        "batching": leg_batching, "serving": leg_serving}
 
 
 def copy_probe(mib: int, iters: int = 20) -> None:

@@ -164,3 +164,20 @@ def test_batch_mixed_with_plain_calls_on_other_grids(double):
         k(*gp, cfg)
     _assert_same(gb, gp)
     batch.close()
+
+
+@pytest.mark.parametrize("precision", ["double", "float"])
+def test_multisolver_mixed_kinds_equal_plain(precision):
+    from xgrid_amd import workloads as W
+    from xgrid_amd.multi import MultiSolver
+    specs = W.batch_specs(8, seed=5, lo=40, hi=160)
+    assert {s.kind for s in specs} == {"cavity", "diffusion"}
+    ms = MultiSolver(precision)
+    handles = [ms.add(s) for s in specs]
+    ms.step(6)
+    for h, s in zip(handles, specs):
+        grids, scalars = ms.grids_for(s)
+        for _ in range(6):
+            ms.plain_call(s, grids, scalars)
+        _assert_same(ms.grids(h), grids)
+    ms.close()

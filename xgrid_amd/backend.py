@@ -55,6 +55,7 @@ class HipKernel:
 
     def __init__(self, op):
         cfg = get_config()
+        self.cfg = cfg                   # options are those in force when the kernel was built
         self.op = op
         self.lib = _lib.ensure_device(cfg.device)
         low, info = lower_kernel(op)
@@ -170,7 +171,7 @@ class HipKernel:
 
     def set_options(self, graphs: bool | None = None, fuse: bool | None = None, blocked: bool | None = None,
                     plan: int | None = None):
-        cfg = get_config()
+        cfg = self.cfg
         g = cfg.graphs and self.lowered.graphs_ok if graphs is None else graphs
         f = cfg.fuse if fuse is None else fuse
         b = cfg.blocked if blocked is None else blocked

@@ -24,7 +24,7 @@ from .backend import build_kernel
 from .grid import Grid
 
 
-class _Precision:
+class precision_scope:
     """Temporarily select what ``float`` means (kernel parse/build and Grid dtype)."""
 
     def __init__(self, precision: str):
@@ -67,7 +67,7 @@ class CavityTwin:
         masks = W.cavity_masks(n)
         self.kernels, self.fields = {}, {}
         for prec in ("double", "float"):
-            with _Precision(prec):
+            with precision_scope(prec):
                 self.kernels[prec] = _built(W.make_cavity())
                 grids = {}
                 for c in self.FIELDS:
@@ -108,7 +108,7 @@ class Heat3DTwin:
         self.nu = nu
         self.kernels, self.fields = {}, {}
         for prec, dt_np in (("double", np.float64), ("float", np.float32)):
-            with _Precision(prec):
+            with precision_scope(prec):
                 u0, mask, dx = W.heat_3d_setup(n, dt_np, seed=seed)
                 self.dx = dx
                 self.dt = 0.1 * dx * dx / nu

@@ -1,0 +1,163 @@
+"""Continuous-batching serving loop for stencil solves.
+
+No counterpart in the reference.  Requests (``workloads.InstanceSpec`` with an
+arrival time and a step count) arrive over time; the server
+
+* admits queued requests in arrival order while their device footprint fits
+  the HBM budget (a request waits at the head of the queue until enough
+  memory is free);
+* keeps every admitted request's fields on the device and advances all of
+  them together, a few kernel calls per tick, in one batch per kernel
+  (``multi.MultiSolver``): requests that arrive while others run join the
+  batch at the next tick, so their work overlaps on the device;
+* retires a request when its steps are done, copies its fields to the host
+  (the response) and frees its memory.
+
+Latency = response ready - arrival.  ``serve_plain`` runs the same trace the
+drop-in way (one request at a time, plain kernel calls) for comparison.
+Each request's result equals the same request solved alone (the batched
+device code per cell is the plain code).
+"""
+from __future__ import annotations
+
+import time
+from collections import deque
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from .multi import MultiSolver
+from .workloads import InstanceSpec
+
+
+def footprint(spec: InstanceSpec, elem_bytes: int) -> int:
+    """Device bytes of one request: per grid two time levels and one scratch of
+    fields, plus the int32 mask (a cavity has four grids, diffusion one)."""
+    grids = 4 if spec.kind == "cavity" else 1
+    return grids * spec.n * spec.n * (3 * elem_bytes + 4)
+
+
+@dataclass
+class Result:
+    rid: int
+    arrival: float
+    admitted: float
+    done: float
+    fields: dict = field(default_factory=dict)
+
+    @property
+    def latency(self) -> float:
+        return self.done - self.arrival
+
+
+def _summary(results: list, wall: float, busy: float) -> dict:
+    lat = np.array([r.latency for r in results])
+    wait = np.array([r.admitted - r.arrival for r in results])
+    return {"requests": len(results), "wall_s": wall, "solves_per_s": len(results) / wall if wall else 0.0,
+            "latency_p50_s": float(np.percentile(lat, 50)), "latency_p99_s": float(np.percentile(lat, 99)),
+            "latency_mean_s": float(lat.mean()), "queue_wait_p50_s": float(np.percentile(wait, 50)),
+            "gpu_busy_frac": busy / wall if wall else 0.0}
+
+
+class Server:
+    def __init__(self, precision: str = "float", hbm_budget: int = 16 << 30, steps_per_tick: int = 8,
+                 graphs: bool = False, keep_fields: bool = True, solver=None, sync=None):
+        self.ms = solver if solver is not None else MultiSolver(precision, graphs=graphs)
+        self.elem = 4 if precision == "float" else 8
+        self.budget = int(hbm_budget)
+        self.steps_per_tick = int(steps_per_tick)
+        self.keep_fields = keep_fields
+        self.sync = sync if sync is not None else (lambda: _lib.check(_lib.load().xg_synchronize()))
+        self.peak_used = 0
+        self.peak_active = 0
+        self.ticks = 0
+
+    def run(self, trace: list, time_scale: float = 1.0) -> tuple:
+        """Serve `trace` (arrival times in seconds, scaled by `time_scale`) in real time;
+        returns (results by request id, summary)."""
+        pending = deque(sorted(trace, key=lambda s: s.arrival))
+        queue: deque = deque()
+        active = {}                      # handle -> [spec, steps done, admitted time]
+        used = 0
+        results = {}
+        busy = 0.0
+        for s in trace:
+            if footprint(s, self.elem) > self.budget:
+                raise ValueError(f"request {s.rid} needs more than the whole HBM budget")
+        self.peak_used = self.peak_active = self.ticks = 0
+        t0 = time.perf_counter()
+        now = lambda: time.perf_counter() - t0        # noqa: E731
+        while pending or queue or active:
+            t = now()
+            while pending and pending[0].arrival * time_scale <= t:
+                queue.append(pending.popleft())
+            while queue and used + footprint(queue[0], self.elem) <= self.budget:
+                s = queue.popleft()
+                h = self.ms.add(s)
+                used += footprint(s, self.elem)
+                active[h] = [s, 0, now()]
+                self.peak_used = max(self.peak_used, used)
+                self.peak_active = max(self.peak_active, len(active))
+            if not active:
+                if pending:
+                    time.sleep(max(0.0, pending[0].arrival * time_scale - now()))
+                continue
+            k = max(0, min(self.steps_per_tick, min(s.steps - d for s, d, _ in active.values())))
+            if k:
+                tb = now()
+                self.ms.step(k)
+                self.sync()
+                busy += now() - tb
+                self.ticks += 1
+            finished = []
+            for h, rec in active.items():
+                rec[1] += k
+                if rec[1] >= rec[0].steps:
+                    finished.append(h)
+            for h in finished:
+                s, _, adm = active.pop(h)
+                grids = self.ms.remove(h)
+                fields = {}
+                if self.keep_fields:
+                    names = "bpuv" if s.kind == "cavity" else "u"
+                    fields = {c: g.now.copy() for c, g in zip(names, grids)}
+                del grids
+                used -= footprint(s, self.elem)
+                results[s.rid] = Result(s.rid, s.arrival * time_scale, adm, now(), fields)
+        wall = now()
+        return results, _summary(list(results.values()), wall, busy)
+
+    def close(self) -> None:
+        self.ms.close()
+
+
+def serve_plain(trace: list, precision: str = "float", time_scale: float = 1.0, keep_fields: bool = True,
+                solver: MultiSolver | None = None) -> tuple:
+    """The same trace served one request at a time with plain kernel calls (FIFO)."""
+    ms = solver or MultiSolver(precision)
+    lib = _lib.load()
+    results = {}
+    busy = 0.0
+    t0 = time.perf_counter()
+    now = lambda: time.perf_counter() - t0            # noqa: E731
+    for s in sorted(trace, key=lambda s: s.arrival):
+        wait = s.arrival * time_scale - now()
+        if wait > 0:
+            time.sleep(wait)
+        adm = now()
+        grids, scalars = ms.grids_for(s)
+        for _ in range(s.steps):
+            ms.plain_call(s, grids, scalars)
+        _lib.check(lib.xg_synchronize())
+        busy += now() - adm
+        fields = {}
+        if keep_fields:
+            names = "bpuv" if s.kind == "cavity" else "u"
+            fields = {c: g.now.copy() for c, g in zip(names, grids)}
+        del grids
+        results[s.rid] = Result(s.rid, s.arrival * time_scale, adm, now(), fields)
+    wall = now()
+    if solver is None:
+        ms.close()
+    return results, _summary(list(results.values()), wall, busy)

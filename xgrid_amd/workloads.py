@@ -297,3 +297,74 @@ def box_walls(shape) -> np.ndarray:
     m = np.ones(shape, np.int32)
     m[tuple(slice(1, -1) for _ in shape)] = 0
     return m
+
+
+# --------------------------------------------------------------------------- many instances
+@dataclass
+class InstanceSpec:
+    """One independent solve of a batched or served workload."""
+    kind: str          # "cavity" | "diffusion"
+    n: int             # grid side
+    nu: float
+    dt: float
+    steps: int = 0     # kernel calls of the solve (serving)
+    arrival: float = 0.0   # seconds after the start of the trace (serving)
+    rid: int = 0
+
+
+def stable_dt(kind: str, n: int, nu: float) -> float:
+    """0.2 dx^2 / nu (explicit diffusion limit with margin); the cavity also keeps
+    the lid's CFL number u dt / dx at most 0.2."""
+    dx = 2.0 / (n - 1)
+    dt = 0.2 * dx * dx / nu
+    return min(dt, 0.2 * dx) if kind == "cavity" else dt
+
+
+def batch_specs(count: int = 256, seed: int = 1, lo: int = 64, hi: int = 1024) -> list:
+    """Independent cavity / 2-D diffusion instances: side log-uniform in [lo, hi],
+    nu uniform in [0.01, 0.1], kind a fair coin, stable dt."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        n = int(round(float(np.exp(rng.uniform(np.log(lo), np.log(hi))))))
+        nu = float(rng.uniform(0.01, 0.1))
+        kind = "cavity" if rng.random() < 0.5 else "diffusion"
+        out.append(InstanceSpec(kind, n, nu, stable_dt(kind, n, nu), rid=i))
+    return out
+
+
+def serving_specs(count: int = 400, seed: int = 2, rate: float = 200.0, lo: int = 128, hi: int = 2048,
+                  steps_lo: int = 100, steps_hi: int = 2000) -> list:
+    """A finite Poisson-arrival trace of cavity solves: exponential gaps at `rate`
+    per second, side log-uniform in [lo, hi], nu uniform in [0.001, 0.1], steps
+    uniform in [steps_lo, steps_hi]."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for i in range(count):
+        t += float(rng.exponential(1.0 / rate))
+        n = int(round(float(np.exp(rng.uniform(np.log(lo), np.log(hi))))))
+        nu = float(rng.uniform(0.001, 0.1))
+        steps = int(rng.integers(steps_lo, steps_hi + 1))
+        out.append(InstanceSpec("cavity", n, nu, stable_dt("cavity", n, nu), steps, t, i))
+    return out
+
+
+def instance_grids(spec: InstanceSpec, Grid):
+    """Fresh grids and the scalar arguments of one instance (zero initial fields;
+    diffusion starts from a hat: 2 inside the middle half, 1 elsewhere)."""
+    n = spec.n
+    if spec.kind == "cavity":
+        masks = cavity_masks(n)
+        grids = []
+        for c in "bpuv":
+            g = Grid((n, n), float)
+            g.boundary[...] = masks[c]
+            grids.append(g)
+        return grids, (CavityConfig(1.0, spec.nu, spec.dt, 2 / (n - 1), 2 / (n - 1)),)
+    g = Grid((n, n), float)
+    u = g.now
+    u[...] = 1.0
+    u[n // 4:3 * n // 4, n // 4:3 * n // 4] = 2.0
+    g.boundary[...] = box_walls((n, n))
+    return [g], (spec.nu, spec.dt, 2 / (n - 1))
