@@ -50,6 +50,8 @@ def main():
     ap.add_argument("--batch-max-blocks", type=int, default=0,
                     help="persistent grid cap of batched launches (0: one workgroup per tile)")
     ap.add_argument("--serve-requests", type=int, default=48, help="requests of the serving trace (400 = full)")
+    ap.add_argument("--tune-limit", type=int, default=24,
+                    help="instances the autotuner times per kernel in the batching/serving legs (0: no tuning)")
     ap.add_argument("--serve-budget-gib", type=float, default=16.0)
     ap.add_argument("--serve-tick", type=int, default=8, help="kernel calls per serving tick")
     ap.add_argument("--serve-low-rate", type=float, default=2.0, help="arrival rate of the below-saturation run")
@@ -284,6 +286,7 @@ def leg_batching(args) -> dict:
                        f"seed 1, {K} steps", "cells": cells}
     for prec in ("double", "float"):
         ms = MultiSolver(prec, max_blocks=args.batch_max_blocks)
+        tuning = ms.autotune(specs, limit=args.tune_limit) if args.tune_limit else {}
         handles = [ms.add(s) for s in specs]
         ms.step(warm)
         _lib.check(lib.xg_synchronize())
@@ -312,6 +315,8 @@ def leg_batching(args) -> dict:
             "solves_per_s_batched": len(specs) / t_batch, "solves_per_s_plain": len(specs) / t_plain,
             "cell_updates_per_s_batched": cells * K / t_batch, "ms_per_batched_step": t_batch / K * 1e3,
             "identical_to_plain": bool(same), "counters": ms.counters(),
+            "tile_shapes": {k: list(v) for k, v in ms.shapes.items()},
+            "autotune": {k: {"timings_ms_per_step": v["timings"], "cached": v["cached"]} for k, v in tuning.items()},
             "plain_counters": {k: {c: v - plain_before[k][c] for c, v in op.native.counters().items()}
                                for k, op in ms.ops.items()}}
         ms.close()
@@ -331,6 +336,7 @@ def leg_serving(args) -> dict:
     budget = int(args.serve_budget_gib * (1 << 30))
     trace = W.serving_specs(args.serve_requests, seed=2, rate=200.0)
     srv = Server("float", hbm_budget=budget, steps_per_tick=args.serve_tick)
+    tuning = srv.ms.autotune(trace, limit=args.tune_limit) if args.tune_limit else {}
     got, summ = srv.run(trace)
     summ.update(peak_resident_requests=srv.peak_active, peak_resident_bytes=srv.peak_used, ticks=srv.ticks)
     ref, summ_plain = serve_plain(trace, "float", solver=srv.ms)
@@ -347,7 +353,8 @@ def leg_serving(args) -> dict:
                         f"{args.serve_tick} steps per tick",
             "cell_steps": work, "batched": summ, "plain_fifo": summ_plain,
             "speedup_solves_per_s": summ["solves_per_s"] / summ_plain["solves_per_s"],
-            "identical_to_plain": bool(same),
+            "identical_to_plain": bool(same), "tile_shapes": {k: list(v) for k, v in srv.ms.shapes.items()},
+            "autotune": {k: {"timings_ms_per_step": v["timings"], "cached": v["cached"]} for k, v in tuning.items()},
             "below_saturation": dict(summ_low, rate_per_s=args.serve_low_rate,
                                      workload=f"{len(low)} requests, seed 3, Poisson {args.serve_low_rate}/s")}
 

@@ -181,3 +181,45 @@ def test_multisolver_mixed_kinds_equal_plain(precision):
             ms.plain_call(s, grids, scalars)
         _assert_same(ms.grids(h), grids)
     ms.close()
+
+
+def test_autotuned_batch_equals_plain(double):
+    """The autotuner times every candidate shape on copies of the instances
+    (leaving them untouched) and the tuned batch still equals plain calls."""
+    from xgrid_amd import workloads as W
+    from xgrid_amd.multi import MultiSolver
+    specs = W.batch_specs(10, seed=9, lo=48, hi=200)
+    ms = MultiSolver("double")
+    tuning = ms.autotune(specs, limit=6, use_cache=False)
+    assert "cavity" in tuning and "diffusion" not in tuning        # diffusion has no blocked loop
+    t = tuning["cavity"]
+    assert all(v is not None and v > 0 for v in t["timings"].values())
+    assert ms.shapes["cavity"] == t["shape"]
+    handles = [ms.add(s) for s in specs]
+    ms.step(5)
+    for h, s in zip(handles, specs):
+        grids, scalars = ms.grids_for(s)
+        for _ in range(5):
+            ms.plain_call(s, grids, scalars)
+        _assert_same(ms.grids(h), grids)
+    ms.close()
+
+
+def test_batched_register_layout_equals_plain(double):
+    from xgrid_amd import workloads as W
+    from xgrid_amd.backend import HipKernel
+    from xgrid_amd.batch import Batch
+    kb, kp = W.make_cavity(), W.make_cavity()
+    batch = Batch(kb, native=HipKernel(kb, (40, 64, 512, 6, "reg")))
+    members = []
+    for n in (65, 130, 41, 200):
+        gb, cfg = _cavity(double, n)
+        ga, _ = _cavity(double, n)
+        batch.add(*gb, cfg)
+        members.append((gb, ga, cfg))
+    batch.step(9)
+    for gb, ga, cfg in members:
+        for _ in range(9):
+            kp(*ga, cfg)
+        _assert_same(gb, ga)
+    batch.close()

@@ -371,3 +371,53 @@ def test_blocked_loop_declined_when_halo_too_small(double):
     run_pair(creep, (du, dv), (hu, hv), steps=2)
     assert creep.native.loop_info(0)["state"] == -1
     assert_bitexact(du, hu)
+
+
+@pytest.mark.parametrize("shape", [(40, 64, 512, 6, "reg"), (24, 64, 512, 4, "reg"), (48, 64, 768, 10, "reg"),
+                                   (32, 64, 512, 8, "lds"), (24, 64, 256, 6, "lds")])
+def test_blocked_loop_shapes_and_layouts_match_oracle(double, shape):
+    """Every tile shape and both layouts (region in LDS / columns in registers)
+    of the blocked loop give the oracle's bits, on grids whose edges cut tiles."""
+    from xgrid_amd import workloads as W
+    from xgrid_amd.backend import build_kernel
+    for n, steps in ((97, 3), (130, 2)):
+        cfg = W.cavity_config(n, dt=2e-6)
+        masks = W.cavity_masks(n)
+        k = W.make_cavity()
+        k.native, k.depth = build_kernel(k, shape)
+        assert k.native.lowered.loops and k.native.lowered.loops[0].variant == shape[4]
+        pairs = {c: grid_pair(xgrid, (n, n), float, masks[c]) for c in "bpuv"}
+        run_pair(k, [pairs[c][0] for c in "bpuv"] + [cfg], [pairs[c][1] for c in "bpuv"] + [cfg], steps=steps)
+        for c in "bpuv":
+            assert_bitexact(*pairs[c], levels=[0])
+        info = k.native.loop_info(0)
+        assert info["state"] == 1 and info["iterations"] == 50 * steps
+
+
+def test_register_layout_declines_when_halo_too_small(double):
+    """The register layout's validation declines the same too-fast boundary
+    creep the LDS layout declines, and the fallback keeps the oracle's bits."""
+    from xgrid_amd.backend import build_kernel
+
+    @xgrid.kernel()
+    def creep2(u: f2, v: f2) -> None:
+        for _ in range(0, 12):
+            u[0, 0] = (u[0, 1][0] + u[0, -1][0] + u[1, 0][0] + u[-1, 0][0]) * 0.25 + v[0, 0]
+            with xgrid.boundary(1):
+                u[0, 0] = u[0, -1][0] + 1.0
+            with xgrid.boundary(2):
+                u[0, 0] = u[0, -1][0] * 0.5
+            with xgrid.boundary(3):
+                u[0, 0] = u[0, -1][0] - 0.25
+
+    creep2.native, creep2.depth = build_kernel(creep2, (40, 64, 512, 6, "reg"))
+    n = 150
+    mask = np.zeros((n, n), np.int32)
+    mask[0, :] = 1 + np.arange(n) % 3
+    mask[-1, :] = mask[:, 0] = mask[:, -1] = 4
+    rng = np.random.default_rng(12)
+    du, hu = grid_pair(xgrid, (n, n), float, mask, rng.random((n, n)))
+    dv, hv = grid_pair(xgrid, (n, n), float, None, rng.random((n, n)) * 1e-3)
+    run_pair(creep2, (du, dv), (hu, hv), steps=2)
+    assert creep2.native.loop_info(0)["state"] == -1
+    assert_bitexact(du, hu)

@@ -28,12 +28,12 @@ class KernelError(RuntimeError):
     pass
 
 
-def lower_kernel(op) -> Lowered:
+def lower_kernel(op, block_shape: tuple | None = None) -> Lowered:
     cfg = get_config()
     info = analyze(op.ir)
     if not cfg.parallel and any(st.implicit for st in info.stmts):
         raise KernelError("Unable to perform implicit operation while parallel is off")
-    return Emitter(info, cfg.overstep, cfg.comment, macro=op.macro).lower(), info
+    return Emitter(info, cfg.overstep, cfg.comment, macro=op.macro, block_shape=block_shape).lower(), info
 
 
 def _ctype_of(t):
@@ -53,12 +53,13 @@ def packed_args_type(arg_types):
 class HipKernel:
     """A compiled kernel: code object, stub entry point and launch plan."""
 
-    def __init__(self, op):
+    def __init__(self, op, block_shape: tuple | None = None):
         cfg = get_config()
         self.cfg = cfg                   # options are those in force when the kernel was built
         self.op = op
         self.lib = _lib.ensure_device(cfg.device)
-        low, info = lower_kernel(op)
+        low, info = lower_kernel(op, block_shape)
+        self.block_shape = Emitter.block_shape() if block_shape is None else tuple(block_shape)
         self.lowered = low
         self.info = info
         self.depth = info.depth
@@ -266,6 +267,6 @@ class HipKernel:
         return result
 
 
-def build_kernel(op):
-    k = HipKernel(op)
+def build_kernel(op, block_shape: tuple | None = None):
+    k = HipKernel(op, block_shape)
     return k, k.depth

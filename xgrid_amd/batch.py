@@ -45,13 +45,17 @@ class _Member:
 class Batch:
     """Instances of one kernel advanced together; see the module docstring."""
 
-    def __init__(self, op, graphs: bool = True, max_blocks: int = 0):
+    def __init__(self, op, graphs: bool = True, max_blocks: int = 0, native=None):
+        """`native`: a build of `op` to run (e.g. with a tuned tile shape); default op's own."""
         if op.mode != "kernel":
             raise BatchError(f"'{op.name}' is not a kernel")
-        if op.native is None:
-            op.native, op.depth = build_kernel(op)
+        if native is None:
+            if op.native is None:
+                op.native, op.depth = build_kernel(op)
+            native = op.native
         self.op = op
-        self.native = op.native
+        self.native = native
+        self.depth = native.depth
         self.lib = _lib.load()
         h = ctypes.c_void_p()
         _lib.check(self.lib.xg_batch_create(self.native.entry_packed, int(bool(op.tick)), ctypes.byref(h)),
@@ -68,7 +72,7 @@ class Batch:
         for g in grids:
             if id(g) in owned:
                 raise BatchError("a grid may belong to one instance of a batch only")
-        depth = self.op.depth
+        depth = self.depth
         handles = []
         for g in grids:
             g._extend_time(depth)
@@ -108,7 +112,7 @@ class Batch:
         """Advance every instance by `steps` kernel calls (asynchronous on the device)."""
         if steps <= 0 or not self.members:
             return
-        depth = self.op.depth
+        depth = self.depth
         for m in self.members.values():
             for g in m.grids:
                 if g.time_levels != depth:

@@ -374,7 +374,7 @@ struct bconst {
   static constexpr bool value = B;
 };
 
-/* ---- temporally blocked loops (2-D) -------------------------------------
+/* ---- temporally blocked loops (2-D), LDS regions --------------------------
  * A workgroup owns a core tile of the grid and keeps core + halo (an RH x RW
  * region) of the loop's grid in LDS for several iterations of the loop body.
  * Region cells outside the grid hold 0 and are never valid.  Only cells at
@@ -387,6 +387,45 @@ __device__ __forceinline__ T tb_ld(const T* lds, const uint8_t* vld, bool& ok, i
   const int i = r * RW + c;
   if (CHECK) ok = ok && vld[i];
   return lds[i];
+}
+
+/* ---- temporally blocked loops (2-D), register columns --------------------
+ * Lane c of a wave holds column c of a 64-column region; a read at column
+ * offset o is the value lane c + o holds, moved across lanes with DPP
+ * (wave_shl:1 moves lane c + 1 to lane c, wave_shr:1 lane c - 1 to lane c;
+ * lanes whose source lies outside the wave get 0).  Every lane of the wave
+ * must execute the move (it reads other lanes' registers). */
+template <int O>
+__device__ __forceinline__ uint32_t lane_shift_u32(uint32_t x) {
+  if (O == 0) return x;
+  int v = (int)x;
+#pragma unroll
+  for (int k = 0; k < (O > 0 ? O : -O); ++k)
+    v = __builtin_amdgcn_update_dpp(0, v, O > 0 ? 0x130 : 0x138, 0xf, 0xf, false);
+  return (uint32_t)v;
+}
+
+template <int O, class T>
+__device__ __forceinline__ T lane_shift(T x) {
+  if (O == 0) return x;
+  if (sizeof(T) == 8) {
+    uint64_t b;
+    __builtin_memcpy(&b, &x, 8);
+    const uint32_t lo = lane_shift_u32<O>((uint32_t)b), hi = lane_shift_u32<O>((uint32_t)(b >> 32));
+    b = ((uint64_t)hi << 32) | lo;
+    T r;
+    __builtin_memcpy(&r, &b, 8);
+    return r;
+  } else if (sizeof(T) == 4) {
+    uint32_t b;
+    __builtin_memcpy(&b, &x, 4);
+    b = lane_shift_u32<O>(b);
+    T r;
+    __builtin_memcpy(&r, &b, 4);
+    return r;
+  } else {
+    return (T)lane_shift_u32<O>((uint32_t)x);
+  }
 }
 
 /* ---- batched launches ---------------------------------------------------

@@ -103,6 +103,7 @@ class LoopMeta:
     kernel: str
     threads: int = 512
     reach: int = 1                # largest |offset| of the body's reads of the loop grid
+    variant: str = "lds"          # "lds": region in LDS; "reg": one column per lane in registers
 
 
 @dataclass
@@ -193,8 +194,9 @@ def function_symbol(op) -> str:
 
 class Emitter:
     def __init__(self, info: KernelInfo, overstep: str = "none", comment: bool = False,
-                 entry: str = "xg_entry", macro: list | None = None):
+                 entry: str = "xg_entry", macro: list | None = None, block_shape: tuple | None = None):
         self.info = info
+        self.shape = tuple(block_shape) if block_shape is not None else self.block_shape()
         self.macro = list(macro or [])
         self.func = info.func
         self.overstep = overstep
@@ -203,6 +205,7 @@ class Emitter:
         self.functions = []            # user operators reachable from the kernel (dependency order)
         self.device_functions = []     # those reachable from stencil expressions
         self._prefetched = set()       # blocked-loop statements whose other-grid loads are prefetched
+        self._tb_reads = {}            # blocked-loop statement -> its loop-grid read offsets, in slot order
         self._collect_functions()
 
     # ------------------------------------------------------------- helpers
@@ -264,12 +267,19 @@ class Emitter:
         T = device_type_name(st.gtype.element)
         grids = {n: t for _, n, t in self.func.grid_args}
         in_lds = [False]     # while lowering value_tb: the loop grid's level 0 lives in LDS
+        in_reg = [False]     # while lowering value_rg: the loop grid's level 0 lives in registers
         pre_mode = [False]   # ... and the other loads come from a per-cell prefetch
         pre_slots: list = []  # (grid, level, offsets) of each prefetched load
+        rd_slots: list = []   # distinct (row, column) offsets of the loop grid's level-0 reads
 
         def cell(e: ir.Cell) -> str:
             if in_lds[0] and e.grid.name == st.grid and e.level == 0:
                 return f"xg::tb_ld<T, CHECK, RHH, RWW>(lds, vld, ok, lr + {e.offsets[0]}, lc + {e.offsets[1]})"
+            if in_reg[0] and e.grid.name == st.grid and e.level == 0:
+                key = (e.offsets[0], e.offsets[1])
+                if key not in rd_slots:
+                    rd_slots.append(key)
+                return f"s[{rd_slots.index(key)}]"
             if pre_mode[0]:
                 key = (e.grid.name, e.level, tuple(e.offsets))
                 if key not in pre_slots:
@@ -292,17 +302,26 @@ class Emitter:
             return f"R.v[{r}].shape[{_ExprC(True, cell, name, grid_info)(e.dim)}]"
 
         expr = _ExprC(True, cell, name, grid_info)(st.node.value)
-        expr_tb = expr_pre = None
+        expr_tb = expr_pre = expr_lds = expr_lds_pre = None
         if tb is not None:
-            in_lds[0] = True
+            in_reg[0] = True
             expr_tb = _ExprC(True, cell, name, grid_info)(st.node.value)
+            in_reg[0] = False
+            in_lds[0] = True
+            expr_lds = _ExprC(True, cell, name, grid_info)(st.node.value)
+            in_lds[0] = False
             if st.klass == 0:
                 pre_mode[0] = True
+                in_reg[0] = True
                 expr_pre = _ExprC(True, cell, name, grid_info)(st.node.value)
+                in_reg[0] = False
+                in_lds[0] = True
+                expr_lds_pre = _ExprC(True, cell, name, grid_info)(st.node.value)
+                in_lds[0] = False
                 pre_mode[0] = False
                 if len(pre_slots) > self.PREFETCH_MAX:
-                    expr_pre = None
-            in_lds[0] = False
+                    expr_pre = expr_lds_pre = None
+            self._tb_reads[sid] = list(rd_slots)
         so = list(st.store_offsets) + [0] * (3 - len(st.store_offsets))
         rl, rh = [0, 0, 0], [0, 0, 0]
         for n in ir.walk(st.node.value):
@@ -331,14 +350,24 @@ class Emitter:
         out.append(f"    return {expr};")
         out.append("  }")
         if expr_tb is not None:
-            # the same expression with the loop grid's level 0 read from the workgroup's LDS region
+            # the same expression with the loop grid's level-0 reads taken from s[], the values
+            # at the statement's distinct offsets (gathered from registers by the blocked kernel)
+            out.append(f"  static constexpr int NRD = {max(1, len(rd_slots))};")
+            out.append("  template <bool SAFE>")
+            out.append("  __device__ static __forceinline__ T value_rg(const T* s, const Refs& R, const Params& P, "
+                       "const int32_t* sh, int i0, int i1, int i2) {")
+            out.append(f"    const int64_t xc = SAFE ? xg::lin<{D}>(sh, i0, i1, i2) : 0;")
+            out.append("    (void)s; (void)R; (void)P; (void)sh; (void)xc; (void)i0; (void)i1; (void)i2;")
+            out.append(f"    return {expr_tb};")
+            out.append("  }")
+            # ... and read from the workgroup's LDS region
             out.append("  template <bool SAFE, bool CHECK, int RHH, int RWW>")
             out.append("  __device__ static __forceinline__ T value_tb(const T* lds, const uint8_t* vld, bool& ok, "
                        "int lr, int lc, const Refs& R, const Params& P, const int32_t* sh, int i0, int i1, int i2) {")
             out.append(f"    const int64_t xc = SAFE ? xg::lin<{D}>(sh, i0, i1, i2) : 0;")
             out.append("    (void)lds; (void)vld; (void)ok; (void)lr; (void)lc; (void)R; (void)P; (void)sh; (void)xc;"
                        " (void)i0; (void)i1; (void)i2;")
-            out.append(f"    return {expr_tb};")
+            out.append(f"    return {expr_lds};")
             out.append("  }")
         if expr_pre is not None:
             # loads of grids the loop does not write, fetched once per launch per cell
@@ -358,11 +387,16 @@ class Emitter:
                 out.append(f"    pre.v{i} = xg::ld<{device_type_name(gt.element)}, {gt.dimension}, XG_OVERSTEP, SAFE>"
                            f"(R.v[{r}], sh, xc, {', '.join(o3)}, i0, i1, i2);")
             out.append("  }")
+            out.append("  __device__ static __forceinline__ T value_rgp(const T* s, const Refs& R, const Params& P, "
+                       "const Pre& pre) {")
+            out.append("    (void)s; (void)R; (void)P; (void)pre;")
+            out.append(f"    return {expr_pre};")
+            out.append("  }")
             out.append("  template <bool CHECK, int RHH, int RWW>")
             out.append("  __device__ static __forceinline__ T value_pre(const T* lds, const uint8_t* vld, bool& ok, "
                        "int lr, int lc, const Refs& R, const Params& P, const Pre& pre) {")
             out.append("    (void)lds; (void)vld; (void)ok; (void)lr; (void)lc; (void)R; (void)P; (void)pre;")
-            out.append(f"    return {expr_pre};")
+            out.append(f"    return {expr_lds_pre};")
             out.append("  }")
             self._prefetched.add(sid)
         out.append("};")
@@ -394,7 +428,7 @@ class Emitter:
         self._stmts_text(f.body, out, expr, 1, device=True)
         out.append("}")
 
-    def _tb_kernel(self, L: LoopMeta, out: list):
+    def _tb_kernel_lds(self, L: LoopMeta, out: list):
         """Blocked kernel of one loop: each workgroup keeps core + halo of the
         loop grid in LDS and runs up to `tmax` iterations of the body on it.
         Every statement is computed for all its cells in the region (reading
@@ -596,6 +630,175 @@ class Emitter:
               f"{{ xg::for_batch_tiles(E, n, total, [&](const xg_btent& e, uint32_t t) "
               f"{{ {L.kernel}_body<{chk}>({bcall}); __syncthreads(); }}); }}")
 
+    def _tb_kernel_reg(self, L: LoopMeta, out: list):
+        """Blocked kernel of one loop.  A workgroup owns a core tile of the loop grid
+        and keeps core + halo (a region RH rows x 64 columns) in registers for up to
+        `tmax` iterations of the loop body: lane c of wave w holds column c of the
+        wave's band of BR rows, plus `reach` rows above and below.  A read at
+        column offset o comes from lane c + o (a cross-lane DPP move); a read at row
+        offset o from the lane's own registers, after the bands' edge rows have been
+        exchanged through LDS (once per statement that reads other rows).  Each
+        statement is evaluated for every region cell from the values before it, then
+        stored where it applies (its class, inside the shrinking exact window), so a
+        statement sees exactly the values the statements before it produced.  The
+        checking variant carries an exactness bit per cell the same way."""
+        T = device_type_name(L.stmts[0].stmt.gtype.element)
+        c0, c1 = L.core
+        m = len(L.stmts)
+        RE = L.reach
+        params = ["const xg_tb_hdr& TH"] + [f"const xg_r{s.stmt.sid}& R{j}, const xg_p{s.stmt.sid}& P{j}"
+                                            for j, s in enumerate(L.stmts)]
+        # reach of the loads that come from global memory (other grids / levels)
+        rl, rh = [0, 0], [0, 0]
+        for s_ in L.stmts:
+            for n in ir.walk(s_.stmt.node.value):
+                if isinstance(n, ir.Cell) and not (n.grid.name == L.grid and n.level == 0):
+                    for d in range(2):
+                        rl[d] = max(rl[d], -n.offsets[d])
+                        rh[d] = max(rh[d], n.offsets[d])
+        o = out.append
+        o("template <bool CHECK>")
+        o(f"__device__ __forceinline__ void {L.kernel}_body({', '.join(params)}, const int tile) {{")
+        o(f"  typedef {T} T;")
+        o(f"  constexpr int TBT = {L.threads}, WV = TBT / 64;")
+        o(f"  constexpr int C0 = {c0}, C1 = {c1}, HL = {L.halo}, RH = C0 + 2 * HL, RW = 64, RE = {RE};")
+        o("  constexpr int BR = RH / WV, NQ = BR + 2 * RE;")
+        o('  static_assert(C1 + 2 * HL == RW && RH % WV == 0, "region: 64 columns, whole rows per wave");')
+        # band-edge rows of every wave, double buffered (alternate exchanges use alternate halves)
+        o("  __shared__ T ex[2][WV][2][RE][64];")
+        o("  __shared__ uint8_t exv[CHECK ? 2 : 1][WV][2][RE][64];")
+        o("  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;")
+        o("  const int32_t* sh = TH.src.shape;")
+        o("  const int N0 = sh[0], N1 = sh[1];")
+        o("  const int nt1 = (N1 + C1 - 1) / C1;")
+        o("  const int g0 = (tile / nt1) * C0, g1 = (tile % nt1) * C1;")
+        o("  const int r0 = g0 - HL, cg = g1 - HL + lane;   /* region origin row; this lane's grid column */")
+        o("  const int br = w * BR;                          /* the band's first region row */")
+        o("  const T* __restrict__ src = static_cast<const T*>(TH.src.ptr);")
+        o("  T V[NQ];")
+        o("  uint8_t OK[CHECK ? NQ : 1];")
+        o("  uint32_t MB[BR];")
+        o("  uint32_t mine_any = 0u;")
+        o("#pragma unroll")
+        o("  for (int q = 0; q < NQ; ++q) {")
+        o("    const int gr = r0 + br + q - RE;")
+        o("    const bool in = gr >= 0 && gr < N0 && cg >= 0 && cg < N1;")
+        o("    const int64_t c = (int64_t)gr * N1 + cg;")
+        o("    V[q] = in ? src[c] : T(0);")
+        o("    if (CHECK) OK[q] = in ? 1 : 0;")
+        o("    if (q >= RE && q < RE + BR) {")
+        o("      uint32_t mb = 0u;")
+        o("      if (in) {")
+        o("        const int32_t k = TH.mask ? TH.mask[c] : 0;")
+        o("        mb = " + " | ".join(f"(k == TH.klass[{j}] ? {1 << j}u : 0u)" for j in range(m)) + ";")
+        o("      }")
+        o("      MB[q - RE] = mb;")
+        o("      mine_any |= mb;")
+        o("    }")
+        o("  }")
+        o("  const uint32_t any = xg::block_or(mine_any);")
+        o(f"  const bool safe = TH.uniform && r0 - {rl[0]} >= 0 && r0 + RH + {rh[0]} <= N0 && "
+          f"g1 - HL - {rl[1]} >= 0 && g1 - HL + RW + {rh[1]} <= N1;")
+        o("  int xb = 0;                 /* exchange buffer half */")
+        # refresh the reach rows above and below the band from the neighbouring bands
+        o("  auto exchange = [&]() {")
+        o("#pragma unroll")
+        o("    for (int k = 0; k < RE; ++k) {")
+        o("      ex[xb][w][0][k][lane] = V[RE + k];")
+        o("      ex[xb][w][1][k][lane] = V[BR + k];")
+        o("      if (CHECK) { exv[CHECK ? xb : 0][w][0][k][lane] = OK[RE + k]; "
+          "exv[CHECK ? xb : 0][w][1][k][lane] = OK[BR + k]; }")
+        o("    }")
+        o("    __syncthreads();")
+        o("#pragma unroll")
+        o("    for (int k = 0; k < RE; ++k) {")
+        o("      if (w > 0) { V[k] = ex[xb][w - 1][1][k][lane]; if (CHECK) OK[k] = exv[CHECK ? xb : 0][w - 1][1][k][lane]; }")
+        o("      if (w < WV - 1) { V[RE + BR + k] = ex[xb][w + 1][0][k][lane]; "
+          "if (CHECK) OK[RE + BR + k] = exv[CHECK ? xb : 0][w + 1][0][k][lane]; }")
+        o("    }")
+        o("    xb ^= 1;")
+        o("  };")
+        o("  auto iterate = [&](auto safe_c) {")
+        o("  constexpr bool SAFE = decltype(safe_c)::value;")
+        for j, s_ in enumerate(L.stmts):
+            if s_.stmt.sid in self._prefetched:
+                S = f"xg_s{s_.stmt.sid}"
+                o(f"  typename {S}::Pre pre{j}[BR];")
+                o("#pragma unroll")
+                o("  for (int q = 0; q < BR; ++q)")
+                o(f"    if (MB[q] & {1 << j}u) {S}::template prefetch<SAFE>(pre{j}[q], R{j}, sh, r0 + br + q, cg, 0);")
+        o("  for (int it = 0; it < TH.iters; ++it) {")
+        for j, s_ in enumerate(L.stmts):
+            S = f"xg_s{s_.stmt.sid}"
+            bit = f"{1 << j}u"
+            reads = self._tb_reads[s_.stmt.sid]
+            vertical = any(r[0] != 0 for r in reads)
+            o(f"    if (any & {bit}) {{")
+            if vertical:
+                o("      exchange();")
+            if s_.stmt.klass == 0:
+                # after `it` iterations only cells (it + 1) * reach inside the region can still be exact
+                o("      const int lo = RE * (it + 1) < RH / 2 ? RE * (it + 1) : RH / 2;")
+            else:
+                o("      const int lo = RE;")
+            o("      T NV[BR];")
+            o("      uint8_t NOK[CHECK ? BR : 1];")
+            o("#pragma unroll")
+            o("      for (int q = 0; q < BR; ++q) {")
+            o("        const int lr = br + q;")
+            o(f"        const bool mine = (MB[q] & {bit}) != 0;")
+            o("        const bool act = lr >= lo && lr < RH - lo && lane >= lo && lane < RW - lo;")
+            # every lane gathers its reads unconditionally (cross-lane moves need the whole wave)
+            o(f"        T s[{S}::NRD];")
+            o("        uint32_t okr = 1u;")
+            for k, (d0, d1) in enumerate(reads):
+                o(f"        s[{k}] = xg::lane_shift<{d1}>(V[q + RE + ({d0})]);")
+                o(f"        if (CHECK) okr &= xg::lane_shift_u32<{d1}>((uint32_t)OK[q + RE + ({d0})]);")
+            if not reads:
+                o("        s[0] = T(0);")
+            if s_.stmt.sid in self._prefetched:
+                o(f"        const T val = {S}::value_rgp(s, R{j}, P{j}, pre{j}[q]);")
+            else:
+                o(f"        const T val = {S}::template value_rg<SAFE>(s, R{j}, P{j}, sh, r0 + lr, cg, 0);")
+            o("        NV[q] = (mine && act) ? val : V[q + RE];")
+            o("        if (CHECK) NOK[q] = (mine && act) ? (uint8_t)(okr != 0u) : (mine ? (uint8_t)0 : OK[q + RE]);")
+            o("      }")
+            o("#pragma unroll")
+            o("      for (int q = 0; q < BR; ++q) {")
+            o("        V[q + RE] = NV[q];")
+            o("        if (CHECK) OK[q + RE] = NOK[q];")
+            o("      }")
+            o("    }")
+        o("  }")
+        o("  };")
+        o("  if (safe) iterate(xg::bconst<true>()); else iterate(xg::bconst<false>());")
+        o("  T* __restrict__ dst = static_cast<T*>(TH.dst.ptr);")
+        o("#pragma unroll")
+        o("  for (int q = 0; q < BR; ++q) {")
+        o("    const int lr = br + q, gr = r0 + lr;")
+        o("    if (lr >= HL && lr < HL + C0 && lane >= HL && lane < HL + C1 && gr < N0 && cg < N1) {")
+        o("      dst[(int64_t)gr * N1 + cg] = V[q + RE];")
+        o("      if (CHECK && !OK[q + RE]) *TH.fail = 1;")
+        o("    }")
+        o("  }")
+        o("}")
+        kargs = ["const xg_tb_hdr TH"] + [f"const xg_r{s_.stmt.sid} R{j}, const xg_p{s_.stmt.sid} P{j}"
+                                          for j, s_ in enumerate(L.stmts)]
+        call = ", ".join(["TH"] + [f"R{j}, P{j}" for j in range(m)] + ["(int)xg::xcd_remap(blockIdx.x, gridDim.x)"])
+        o(f'extern "C" __global__ void __launch_bounds__({L.threads}) {L.kernel}({", ".join(kargs)}) '
+          f"{{ {L.kernel}_body<false>({call}); }}")
+        o(f'extern "C" __global__ void __launch_bounds__({L.threads}) {L.kernel}_chk({", ".join(kargs)}) '
+          f"{{ {L.kernel}_body<true>({call}); }}")
+        # batched: the tiles of many instances; the barrier lets a workgroup reuse its LDS for the next tile
+        bcall = ", ".join(["e.th"] + [f"*static_cast<const xg_r{s_.stmt.sid}*>(e.refs[{j}]), "
+                                      f"*static_cast<const xg_p{s_.stmt.sid}*>(e.params[{j}])"
+                                      for j, s_ in enumerate(L.stmts)] + ["(int)t"])
+        for suffix, chk in (("_b", "false"), ("_chkb", "true")):
+            o(f'extern "C" __global__ void __launch_bounds__({L.threads}) {L.kernel}{suffix}('
+              f"const xg_btent* __restrict__ E, const int32_t n, const int64_t total) "
+              f"{{ xg::for_batch_tiles(E, n, total, [&](const xg_btent& e, uint32_t t) "
+              f"{{ {L.kernel}_body<{chk}>({bcall}); __syncthreads(); }}); }}")
+
     def device_source(self, stmt_meta: list, groups: list, loops: list = ()) -> str:
         out = ["// generated by xgrid_amd for the MI355X backend", '#include "xgrid_device.h"',
                f"#define XG_OVERSTEP {OVERSTEP_CODE[self.overstep]}"]
@@ -612,7 +815,7 @@ class Emitter:
         for m in stmt_meta:
             self._device_stmt(m.stmt, m.refs, m.params, out, loop_of.get(m.stmt.sid))
         for L in loops:
-            self._tb_kernel(L, out)
+            (self._tb_kernel_reg if L.variant == "reg" else self._tb_kernel_lds)(L, out)
         # phase-2 kernels per (element type, dimension)
         done = set()
         for m in stmt_meta:
@@ -875,16 +1078,26 @@ class Emitter:
 
     PREFETCH_MAX = 4   # most other-grid loads of a blocked statement kept in registers per cell
 
-    @staticmethod
-    def block_shape() -> tuple:
-        """(core rows, region columns, work-items, most iterations per launch) of blocked
-        loops.  The region (core + halo) is a power of two wide so a region index splits
-        into row and column with shifts; the core is the region width less the halo.
-        XGRID_TB_SHAPE="rows,region_cols,threads,tmax" overrides the default."""
+    DEFAULT_BLOCK_SHAPE = (48, 64, 768, 10, "lds")
+
+    @classmethod
+    def block_shape(cls, spec: str | None = None) -> tuple:
+        """(core rows, region columns, work-items, most iterations per launch, variant)
+        of blocked loops.  The region (core + halo) is a power of two wide so a region
+        index splits into row and column with shifts; the core is the region width less
+        the halo.  Variant "lds" keeps the region in LDS; "reg" keeps one region column
+        per lane in registers (64 columns).  XGRID_TB_SHAPE="rows,cols,threads,tmax[,variant]"
+        overrides the default."""
         import os
-        spec = os.environ.get("XGRID_TB_SHAPE", "48,64,768,10")
-        r, c, t, m = (int(x) for x in spec.split(","))
-        return r, c, t, m
+        spec = spec if spec is not None else os.environ.get("XGRID_TB_SHAPE")
+        if not spec:
+            return cls.DEFAULT_BLOCK_SHAPE
+        parts = spec.split(",")
+        r, c, t, m = (int(x) for x in parts[:4])
+        v = parts[4].strip() if len(parts) > 4 else "lds"
+        if v not in ("lds", "reg"):
+            raise EmitError(f"unknown blocked-loop variant '{v}'")
+        return r, c, t, m, v
 
     def _blockable_loops(self, stmt_meta: list) -> list:
         """`for` loops that may run temporally blocked: the body is 1..8 stencil
@@ -911,17 +1124,24 @@ class Emitter:
                 for c in ir.walk(m.stmt.node.value):
                     if isinstance(c, ir.Cell) and c.grid.name == g and c.level == 0:
                         reach = max(reach, max(abs(o) for o in c.offsets))
-            rows, region_cols, threads, tm = self.block_shape()
+            rows, region_cols, threads, tm, variant = self.shape
             tmax = max(1, tm // max(1, reach)) if reach else tm
             # information moves `reach` cells per iteration through the bulk; a boundary
             # statement reading a neighbour computed in the same iteration adds a hop,
             # so leave room for two such hops (a validation launch proves the choice)
             halo = tmax * reach + 2 * reach
             cols = region_cols - 2 * halo
-            if halo > 16 or cols < 16:
+            if halo > 16 or cols < 16 or threads % 64 or threads > 1024:
                 continue
+            if variant == "reg":
+                waves = threads // 64
+                if region_cols != 64 or reach < 1:
+                    continue
+                rows += (-(rows + 2 * halo)) % waves      # whole region rows per wave
+                if (rows + 2 * halo) // waves > 16:
+                    continue
             loops.append(LoopMeta(len(loops), n, ms, g, (rows, cols), halo, tmax, f"xg_tb{len(loops)}", threads,
-                                  reach))
+                                  reach, variant))
         return loops
 
     def _groups(self, stmt_meta: list) -> list:

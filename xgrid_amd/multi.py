@@ -11,7 +11,7 @@ No counterpart in the reference (one kernel call at a time).
 from __future__ import annotations
 
 from . import workloads as W
-from .backend import build_kernel
+from .backend import HipKernel, build_kernel
 from .batch import Batch
 from .grid import Grid
 from .precision import precision_scope
@@ -28,9 +28,35 @@ class MultiSolver:
                 op = make()
                 op.native, op.depth = build_kernel(op)
                 self.ops[kind] = op
+        self.graphs, self.max_blocks = graphs, max_blocks
         self.batches = {k: Batch(op, graphs=graphs, max_blocks=max_blocks) for k, op in self.ops.items()}
         self.members = {}            # handle -> (kind, member id, grids, spec)
         self._next = 1
+        self.shapes = {k: op.native.block_shape for k, op in self.ops.items()}
+        self.tuning = {}
+
+    def autotune(self, specs: list, limit: int = 48, use_cache: bool = True) -> dict:
+        """Pick each kernel's blocked-loop tile shape on fresh copies of a sample of
+        `specs` (before any instance is added) and rebuild its batch with it."""
+        from . import autotune
+        if self.members:
+            raise RuntimeError("autotune before adding instances")
+        for kind, op in self.ops.items():
+            mine = [s for s in specs if s.kind == kind]
+            if not mine:
+                continue
+            with precision_scope(self.precision):
+                if not autotune.has_blocked_loop(op):
+                    continue
+                res = autotune.tune_batch(op, lambda s: W.instance_grids(s, Grid),
+                                          autotune.sample(mine, limit, size=lambda s: s.n), self.precision,
+                                          max_blocks=self.max_blocks, use_cache=use_cache)
+                native = HipKernel(op, res["shape"])
+            self.batches[kind].close()
+            self.batches[kind] = Batch(op, graphs=self.graphs, max_blocks=self.max_blocks, native=native)
+            self.shapes[kind] = res["shape"]
+            self.tuning[kind] = res
+        return self.tuning
 
     def grids_for(self, spec: W.InstanceSpec):
         with precision_scope(self.precision):
