@@ -25,6 +25,8 @@ typedef __hip_internal::uint64_t uint64_t;
 #endif
 
 #define XG_MAX_DIMS 3
+/* 3-D workgroup tiles: chunks of 4 planes a workgroup does in turn */
+#define XG_TILE3_CHUNKS 1
 #define XG_MAX_REFS 16
 
 /* domain kinds: how a boundary class's cells are enumerated */

@@ -39,14 +39,15 @@
 
 namespace xg {
 
-/* Workgroup tile: 256 work-items, NC cells each.
+/* Workgroup tile: 256 work-items, NC cells each per chunk, CH chunks in turn.
  *   1-D: 1024 consecutive cells (work-item t: t, t+256, t+512, t+768)
  *   2-D: 4 rows x 64 columns (one cell per work-item)
- *   3-D: 4 planes x 4 rows x 64 columns (work-item (r, c): planes 0..3) */
+ *   3-D: XG_TILE3_CHUNKS x 4 planes x 4 rows x 64 columns (work-item (r, c):
+ *        planes 0..3 of each chunk, chunk after chunk) */
 template <int D> struct Tile;
-template <> struct Tile<1> { static constexpr int E0 = 1024, E1 = 1, E2 = 1, NC = 4; };
-template <> struct Tile<2> { static constexpr int E0 = 4, E1 = 64, E2 = 1, NC = 1; };
-template <> struct Tile<3> { static constexpr int E0 = 4, E1 = 4, E2 = 64, NC = 4; };
+template <> struct Tile<1> { static constexpr int E0 = 1024, E1 = 1, E2 = 1, NC = 4, CH = 1; };
+template <> struct Tile<2> { static constexpr int E0 = 4, E1 = 64, E2 = 1, NC = 1, CH = 1; };
+template <> struct Tile<3> { static constexpr int E0 = 4, E1 = 4, E2 = 64, NC = 4, CH = XG_TILE3_CHUNKS; };
 
 /* The hardware hands workgroup b to XCD (b mod 8).  Give each XCD one
  * contiguous run of tiles so vertical neighbours stay in that XCD's L2. */
@@ -159,7 +160,7 @@ struct TileCells {
   bool in[NC];
   int32_t tlo[3], thi[3];
 
-  __device__ __forceinline__ TileCells(const int32_t* lo, const int32_t* hi, uint32_t tile) {
+  __device__ __forceinline__ TileCells(const int32_t* lo, const int32_t* hi, uint32_t tile, int chunk = 0) {
     typedef Tile<D> TL;
     const int tid = threadIdx.x;
     int t0 = (int)tile, t1 = 0, t2 = 0;
@@ -168,14 +169,16 @@ struct TileCells {
       t0 = (int)tile / n1;
       t1 = (int)tile - t0 * n1;
     } else if (D == 3) {
+      /* rows slowest, planes in the middle: tiles that share a halo plane run
+       * a few tiles apart, while the planes are still in the XCD's L2 */
       const int n2 = (hi[2] - lo[2] + TL::E2 - 1) / TL::E2;
-      const int n1 = (hi[1] - lo[1] + TL::E1 - 1) / TL::E1;
-      t0 = (int)tile / (n1 * n2);
-      const int rem = (int)tile - t0 * n1 * n2;
-      t1 = rem / n2;
-      t2 = rem - t1 * n2;
+      const int n0 = (hi[0] - lo[0] + TL::E0 * TL::CH - 1) / (TL::E0 * TL::CH);
+      t1 = (int)tile / (n0 * n2);
+      const int rem = (int)tile - t1 * n0 * n2;
+      t0 = rem / n2;
+      t2 = rem - t0 * n2;
     }
-    tlo[0] = lo[0] + t0 * TL::E0;
+    tlo[0] = lo[0] + t0 * (TL::E0 * TL::CH) + chunk * TL::E0;
     tlo[1] = D > 1 ? lo[1] + t1 * TL::E1 : 0;
     tlo[2] = D > 2 ? lo[2] + t2 * TL::E2 : 0;
     thi[0] = min(tlo[0] + TL::E0, hi[0]);
@@ -199,6 +202,16 @@ struct TileCells {
       in[n] = c[n][0] < thi[0] && (D < 2 || c[n][1] < thi[1]) && (D < 3 || c[n][2] < thi[2]);
     }
   }
+
+  /* Move cells past the tile's edge onto its last cell (in[] keeps which are real). */
+  __device__ __forceinline__ void clamp_to_tile() {
+#pragma unroll
+    for (int n = 0; n < NC; ++n) {
+      c[n][0] = min(c[n][0], thi[0] - 1);
+      if (D > 1) c[n][1] = min(c[n][1], thi[1] - 1);
+      if (D > 2) c[n][2] = min(c[n][2], thi[2] - 1);
+    }
+  }
 };
 
 /* Do the tile's cells, the statement's loads and its store all lie inside the
@@ -218,26 +231,31 @@ __device__ __forceinline__ void box_cells(const xg_launch_hdr& H, const typename
                                           const typename S::Params& P, uint32_t tile) {
   constexpr int D = S::D;
   typedef typename S::T T;
-  TileCells<D> tc(H.dom.lo, H.dom.hi, tile);
-  constexpr int NC = TileCells<D>::NC;
-  if (MASKED) {
+  for (int ch = 0; ch < Tile<D>::CH; ++ch) {
+    TileCells<D> tc(H.dom.lo, H.dom.hi, tile, ch);
+    if (tc.tlo[0] >= H.dom.hi[0]) break;
+    constexpr int NC = TileCells<D>::NC;
+    /* A work-item evaluates all its cells without branching, so their loads are
+     * in flight together: a cell past the tile's edge is evaluated at the
+     * nearest cell inside the tile (a valid address) and not stored. */
+    tc.clamp_to_tile();
+    if (MASKED) {
+#pragma unroll
+      for (int n = 0; n < NC; ++n)
+        tc.in[n] = tc.in[n] && H.dom.mask[lin<D>(H.out.shape, tc.c[n][0], tc.c[n][1], tc.c[n][2])] == H.dom.klass;
+    }
+    T v[NC];
+    if (tile_safe<S>(H, tc.tlo, tc.thi)) {
+#pragma unroll
+      for (int n = 0; n < NC; ++n) v[n] = S::template value<true>(R, P, H.out.shape, tc.c[n][0], tc.c[n][1], tc.c[n][2]);
+    } else {
+#pragma unroll
+      for (int n = 0; n < NC; ++n) v[n] = S::template value<false>(R, P, H.out.shape, tc.c[n][0], tc.c[n][1], tc.c[n][2]);
+    }
 #pragma unroll
     for (int n = 0; n < NC; ++n)
-      tc.in[n] = tc.in[n] && H.dom.mask[lin<D>(H.out.shape, tc.c[n][0], tc.c[n][1], tc.c[n][2])] == H.dom.klass;
+      if (tc.in[n]) store<S, OV>(H, tc.c[n][0], tc.c[n][1], tc.c[n][2], v[n]);
   }
-  T v[NC];
-  if (tile_safe<S>(H, tc.tlo, tc.thi)) {
-#pragma unroll
-    for (int n = 0; n < NC; ++n)
-      if (tc.in[n]) v[n] = S::template value<true>(R, P, H.out.shape, tc.c[n][0], tc.c[n][1], tc.c[n][2]);
-  } else {
-#pragma unroll
-    for (int n = 0; n < NC; ++n)
-      if (tc.in[n]) v[n] = S::template value<false>(R, P, H.out.shape, tc.c[n][0], tc.c[n][1], tc.c[n][2]);
-  }
-#pragma unroll
-  for (int n = 0; n < NC; ++n)
-    if (tc.in[n]) store<S, OV>(H, tc.c[n][0], tc.c[n][1], tc.c[n][2], v[n]);
 }
 
 /* Tile `tile` of `ntiles` of the statement's class (a list class: a slice of
@@ -273,8 +291,11 @@ __device__ void run_pingpong(const xg_launch_hdr& H, const typename S::Refs& R, 
   const int32_t* shape = H.out.shape;
   const T* __restrict__ src = static_cast<const T*>(H.src.ptr);
   T* __restrict__ out = static_cast<T*>(H.out.ptr);
-  TileCells<D> tc(zero, shape, tile);
+  for (int ch = 0; ch < Tile<D>::CH; ++ch) {
+  TileCells<D> tc(zero, shape, tile, ch);
+  if (tc.tlo[0] >= shape[0]) break;
   constexpr int NC = TileCells<D>::NC;
+  tc.clamp_to_tile();          /* evaluate every cell without branching (see box_cells) */
   bool member[NC];
   int64_t lc[NC];
 #pragma unroll
@@ -285,33 +306,41 @@ __device__ void run_pingpong(const xg_launch_hdr& H, const typename S::Refs& R, 
       bool m = c[0] >= dom.lo[0] && c[0] < dom.hi[0];
       if (D > 1) m = m && c[1] >= dom.lo[1] && c[1] < dom.hi[1];
       if (D > 2) m = m && c[2] >= dom.lo[2] && c[2] < dom.hi[2];
-      member[n] = tc.in[n] && m;
+      member[n] = m;
     } else {
-      member[n] = tc.in[n] && dom.mask[lc[n]] == dom.klass;
+      member[n] = dom.mask[lc[n]] == dom.klass;
     }
   }
   T v[NC];
   if (tile_safe<S>(H, tc.tlo, tc.thi)) {
 #pragma unroll
-    for (int n = 0; n < NC; ++n)
-      if (tc.in[n]) v[n] = member[n] ? S::template value<true>(R, P, H.out.shape, tc.c[n][0], tc.c[n][1], tc.c[n][2]) : src[lc[n]];
+    for (int n = 0; n < NC; ++n) {
+      const T x = S::template value<true>(R, P, H.out.shape, tc.c[n][0], tc.c[n][1], tc.c[n][2]);
+      v[n] = member[n] ? x : src[lc[n]];
+    }
   } else {
 #pragma unroll
-    for (int n = 0; n < NC; ++n)
-      if (tc.in[n]) v[n] = member[n] ? S::template value<false>(R, P, H.out.shape, tc.c[n][0], tc.c[n][1], tc.c[n][2]) : src[lc[n]];
+    for (int n = 0; n < NC; ++n) {
+      const T x = S::template value<false>(R, P, H.out.shape, tc.c[n][0], tc.c[n][1], tc.c[n][2]);
+      v[n] = member[n] ? x : src[lc[n]];
+    }
   }
 #pragma unroll
   for (int n = 0; n < NC; ++n)
     if (tc.in[n]) out[lc[n]] = v[n];
+  }
 }
 
 /* Visit every cell of the box [lo, hi) handled by this workgroup's tile. */
 template <int D, class F>
 __device__ __forceinline__ void for_tile(const int32_t* lo, const int32_t* hi, uint32_t tile, F&& f) {
-  TileCells<D> tc(lo, hi, tile);
+  for (int ch = 0; ch < Tile<D>::CH; ++ch) {
+    TileCells<D> tc(lo, hi, tile, ch);
+    if (tc.tlo[0] >= hi[0]) break;
 #pragma unroll
-  for (int n = 0; n < TileCells<D>::NC; ++n)
-    if (tc.in[n]) f(tc.c[n][0], tc.c[n][1], tc.c[n][2]);
+    for (int n = 0; n < TileCells<D>::NC; ++n)
+      if (tc.in[n]) f(tc.c[n][0], tc.c[n][1], tc.c[n][2]);
+  }
 }
 
 /* Phase 2 of a two-phase statement: store the temporary at the shifted cell. */

@@ -677,7 +677,9 @@ class Emitter:
         o("  const T* __restrict__ src = static_cast<const T*>(TH.src.ptr);")
         o("  T V[NQ];")
         o("  uint8_t OK[CHECK ? NQ : 1];")
-        o("  uint32_t MB[BR];")
+        o("  uint32_t MB[(BR + 3) / 4];          /* statement-membership bits: 8 per row */")
+        o("#pragma unroll")
+        o("  for (int q = 0; q < (BR + 3) / 4; ++q) MB[q] = 0u;")
         o("  uint32_t mine_any = 0u;")
         o("#pragma unroll")
         o("  for (int q = 0; q < NQ; ++q) {")
@@ -692,7 +694,7 @@ class Emitter:
         o("        const int32_t k = TH.mask ? TH.mask[c] : 0;")
         o("        mb = " + " | ".join(f"(k == TH.klass[{j}] ? {1 << j}u : 0u)" for j in range(m)) + ";")
         o("      }")
-        o("      MB[q - RE] = mb;")
+        o("      MB[(q - RE) >> 2] |= mb << (8 * ((q - RE) & 3));")
         o("      mine_any |= mb;")
         o("    }")
         o("  }")
@@ -726,7 +728,8 @@ class Emitter:
                 o(f"  typename {S}::Pre pre{j}[BR];")
                 o("#pragma unroll")
                 o("  for (int q = 0; q < BR; ++q)")
-                o(f"    if (MB[q] & {1 << j}u) {S}::template prefetch<SAFE>(pre{j}[q], R{j}, sh, r0 + br + q, cg, 0);")
+                o(f"    if ((MB[q >> 2] >> (8 * (q & 3))) & {1 << j}u) "
+                  f"{S}::template prefetch<SAFE>(pre{j}[q], R{j}, sh, r0 + br + q, cg, 0);")
         o("  for (int it = 0; it < TH.iters; ++it) {")
         for j, s_ in enumerate(L.stmts):
             S = f"xg_s{s_.stmt.sid}"
@@ -741,12 +744,13 @@ class Emitter:
                 o("      const int lo = RE * (it + 1) < RH / 2 ? RE * (it + 1) : RH / 2;")
             else:
                 o("      const int lo = RE;")
-            o("      T NV[BR];")
-            o("      uint8_t NOK[CHECK ? BR : 1];")
+            # new rows wait `reach` rows before they replace the old ones (later rows still read them)
+            o("      T hold[RE];")
+            o("      uint8_t hok[CHECK ? RE : 1];")
             o("#pragma unroll")
             o("      for (int q = 0; q < BR; ++q) {")
             o("        const int lr = br + q;")
-            o(f"        const bool mine = (MB[q] & {bit}) != 0;")
+            o(f"        const bool mine = ((MB[q >> 2] >> (8 * (q & 3))) & {bit}) != 0;")
             o("        const bool act = lr >= lo && lr < RH - lo && lane >= lo && lane < RW - lo;")
             # every lane gathers its reads unconditionally (cross-lane moves need the whole wave)
             o(f"        T s[{S}::NRD];")
@@ -760,13 +764,20 @@ class Emitter:
                 o(f"        const T val = {S}::value_rgp(s, R{j}, P{j}, pre{j}[q]);")
             else:
                 o(f"        const T val = {S}::template value_rg<SAFE>(s, R{j}, P{j}, sh, r0 + lr, cg, 0);")
-            o("        NV[q] = (mine && act) ? val : V[q + RE];")
-            o("        if (CHECK) NOK[q] = (mine && act) ? (uint8_t)(okr != 0u) : (mine ? (uint8_t)0 : OK[q + RE]);")
+            o("        const T nv = (mine && act) ? val : V[q + RE];")
+            o("        const uint8_t nok = CHECK ? ((mine && act) ? (uint8_t)(okr != 0u) : (mine ? (uint8_t)0 : OK[q + RE]))"
+              " : (uint8_t)0;")
+            o("        if (q >= RE) {")
+            o("          V[q] = hold[q % RE];")
+            o("          if (CHECK) OK[q] = hok[CHECK ? q % RE : 0];")
+            o("        }")
+            o("        hold[q % RE] = nv;")
+            o("        if (CHECK) hok[CHECK ? q % RE : 0] = nok;")
             o("      }")
             o("#pragma unroll")
-            o("      for (int q = 0; q < BR; ++q) {")
-            o("        V[q + RE] = NV[q];")
-            o("        if (CHECK) OK[q + RE] = NOK[q];")
+            o("      for (int q = BR - RE; q < BR; ++q) {")
+            o("        V[q + RE] = hold[q % RE];")
+            o("        if (CHECK) OK[q + RE] = hok[CHECK ? q % RE : 0];")
             o("      }")
             o("    }")
         o("  }")
