@@ -15,6 +15,7 @@ namespace xgi {
 constexpr int kThreads = 256;
 constexpr int kGroupThreads = 1024;
 constexpr int64_t kFuseMaxCells = 1 << 16;  // a chained statement walks at most this many cells
+constexpr int64_t kGraphMinLaunches = 4;    // a call with fewer launches runs them directly (cheaper)
 
 // kRecord: the stub's statements are recorded for a batched launch instead of run
 enum CallMode { kDirect = 0, kCapture = 1, kReplay = 2, kRecord = 3 };
@@ -36,6 +37,7 @@ struct GraphEntry {
   uint64_t hash = 0;
   int seen = 0;
   hipGraphExec_t exec = nullptr;
+  bool direct = false;          // the call launches too little for a graph to pay: run it directly
   std::vector<std::vector<void*>> post_levels;
   std::vector<void*> post_scratch;
   uint64_t last_use = 0;
@@ -82,6 +84,7 @@ struct xg_kernel {
   std::vector<xgi::GraphEntry> cache;
   uint64_t tick = 0;
   int64_t n_launch = 0, n_fused = 0, n_replay = 0, n_capture = 0;
+  int64_t capture_mark = 0;         // n_launch when the current capture began
   int32_t timed_sid = -1;
   std::vector<xgi::EventPair> events;
   std::vector<xgi::LoopPlan> loops;

@@ -451,6 +451,10 @@ int api_begin(xg_kernel* k, xg_grid* const* grids, int32_t ngrids, const void* k
       e.last_use = k->tick;
       e.seen++;
       k->cur = &e;
+      if (e.direct) {
+        k->cur = nullptr;
+        return XG_OK;
+      }
       if (e.exec) {
         k->mode = kReplay;
         k->n_replay++;
@@ -460,6 +464,7 @@ int api_begin(xg_kernel* k, xg_grid* const* grids, int32_t ngrids, const void* k
         return XG_OK;
       }
       k->mode = kCapture;
+      k->capture_mark = k->n_launch;
       hipError_t err = hipStreamBeginCapture(g_dev.stream, hipStreamCaptureModeThreadLocal);
       if (err != hipSuccess) {
         k->mode = kDirect;
@@ -573,6 +578,13 @@ int api_end(xg_kernel* k) {
     e = hipGraphLaunch(exec, g_dev.stream);
     k->mode = kDirect;
     if (e != hipSuccess) return set_status(k, fail(XG_ERR_HIP, std::string("hipGraphLaunch: ") + hipGetErrorString(e)));
+    if (k->n_launch - k->capture_mark < kGraphMinLaunches) {
+      // launching this graph costs more host time than launching its few kernels
+      (void)hipStreamSynchronize(g_dev.stream);
+      (void)hipGraphExecDestroy(exec);
+      ent->exec = nullptr;
+      ent->direct = true;
+    }
     return k->status;
   }
   if (k->mode == kReplay) {
