@@ -223,3 +223,101 @@ def test_batched_register_layout_equals_plain(double):
             kp(*ga, cfg)
         _assert_same(gb, ga)
     batch.close()
+
+
+def test_batch_mixes_validated_and_declined_blocked_plans(double):
+    """One batch, two kinds of masks: on one the blocked plan validates, on the
+    other boundary data creeps faster than the halo and the plan is declined;
+    the batched loop runs blocked for the first kind and statement by statement
+    for the second, and every instance still equals its plain run."""
+    import xgrid_amd as xgrid
+    from xgrid_amd.batch import Batch
+    f2 = xgrid.grid[float, 2]
+
+    def make():
+        @xgrid.kernel()
+        def creep(u: f2, v: f2) -> None:
+            for _ in range(0, 12):
+                u[0, 0] = (u[0, 1][0] + u[0, -1][0] + u[1, 0][0] + u[-1, 0][0]) * 0.25 + v[0, 0]
+                with xgrid.boundary(1):
+                    u[0, 0] = u[0, -1][0] + 1.0
+                with xgrid.boundary(2):
+                    u[0, 0] = u[0, -1][0] * 0.5
+                with xgrid.boundary(3):
+                    u[0, 0] = u[0, -1][0] - 0.25
+        return creep
+
+    kb, kp = make(), make()
+    batch = Batch(kb)
+    rng = np.random.default_rng(21)
+    members = []
+    for i, n in enumerate((150, 120, 90, 140)):
+        mask = np.zeros((n, n), np.int32)
+        if i % 2 == 0:
+            mask[0, :] = 1 + np.arange(n) % 3           # creeping along row 0: declined
+        else:
+            mask[0, :] = 1                               # one class on row 0: blocked
+            mask[:, 0] = 2
+        mask[-1, :] = mask[:, -1] = 3
+        u0 = rng.random((n, n))
+        v0 = rng.random((n, n)) * 1e-3
+        pair = []
+        for _ in range(2):
+            u = xgrid.Grid((n, n), float)
+            v = xgrid.Grid((n, n), float)
+            u.now[...] = u0
+            u.boundary[...] = mask
+            v.now[...] = v0
+            pair.append((u, v))
+        batch.add(*pair[0])
+        members.append(pair)
+    batch.step(3)
+    for (ub, vb), (ua, va) in members:
+        for _ in range(3):
+            kp(ua, va)
+        _assert_same([ub, vb], [ua, va])
+    c = batch.counters()
+    assert c["validations"] >= 1
+    batch.close()
+
+
+def test_batch_runs_two_phase_statements(double):
+    """A statement with a shifted store runs compute-then-store; batched, the
+    two phases run for all instances and results equal plain calls."""
+    import xgrid_amd as xgrid
+    from xgrid_amd.batch import Batch
+    f2 = xgrid.grid[float, 2]
+
+    def make():
+        @xgrid.kernel()
+        def shift(u: f2, w: f2, a: float) -> None:
+            u[0, 1] = u[0, 0][0] * a + w[0, 0]       # reads the store grid now: compute, then store
+            with xgrid.boundary(1):
+                w[0, 0] = w[0, 0][0] + u[0, 0][0]
+        return shift
+
+    kb, kp = make(), make()
+    batch = Batch(kb)
+    rng = np.random.default_rng(5)
+    members = []
+    for n, a in ((40, 0.5), (77, 1.5), (64, -0.25)):
+        mask = np.zeros((n, n), np.int32)
+        mask[:, -1] = 1
+        pair = []
+        for _ in range(2):
+            u = xgrid.Grid((n, n), float)
+            w = xgrid.Grid((n, n), float)
+            u.now[...] = rng.random((n, n)) if not pair else pair[0][0].now
+            w.now[...] = rng.random((n, n)) if not pair else pair[0][1].now
+            u.boundary[...] = mask
+            w.boundary[...] = mask
+            pair.append((u, w))
+        batch.add(*pair[0], a)
+        members.append((pair, a))
+    assert kb.native.lowered.stmts[0].stmt.mode == "twophase"
+    batch.step(4)
+    for ((ub, wb), (ua, wa)), a in members:
+        for _ in range(4):
+            kp(ua, wa, a)
+        _assert_same([ub, wb], [ua, wa])
+    batch.close()
