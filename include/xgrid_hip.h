@@ -231,6 +231,12 @@ int xg_grid_poisson_residual(xg_grid* p, int32_t level_p, int32_t type_p, xg_gri
                              double* linf);
 
 /* ---- measurement probes ------------------------------------------------ */
+/* Record an event on the backend stream (a point in its work); elapsed time
+ * between two recorded events (waits for the second); release one.  No
+ * reference counterpart: the serving loop measures device busy time with them. */
+int xg_event_record(void** out);
+int xg_event_elapsed(void* a, void* b, double* ms);
+int xg_event_destroy(void* e);
 /* Device copy of `bytes` bytes repeated `iters` times; returns GB/s (read+write). */
 int xg_copy_probe(int64_t bytes, int32_t iters, double* gbps, double* ms_per_iter);
 

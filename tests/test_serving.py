@@ -35,6 +35,20 @@ class FakeSolver:
         pass
 
 
+class FakeTimer:
+    def begin(self):
+        pass
+
+    def end(self):
+        pass
+
+    def sync(self):
+        pass
+
+    def busy_seconds(self):
+        return 0.0
+
+
 def _trace(n=12):
     out = []
     for i in range(n):
@@ -46,7 +60,7 @@ def _trace(n=12):
 def test_every_request_gets_exactly_its_steps_in_fifo_order():
     trace = _trace()
     fake = FakeSolver()
-    srv = Server("float", hbm_budget=1 << 30, steps_per_tick=4, solver=fake, sync=lambda: None, keep_fields=False)
+    srv = Server("float", hbm_budget=1 << 30, steps_per_tick=4, solver=fake, timer=FakeTimer(), keep_fields=False)
     results, summ = srv.run(trace)
     assert sorted(results) == [s.rid for s in trace]
     for s in trace:
@@ -61,7 +75,7 @@ def test_budget_bounds_the_resident_set():
     trace = _trace(10)
     one = max(footprint(s, 4) for s in trace)
     fake = FakeSolver()
-    srv = Server("float", hbm_budget=2 * one, steps_per_tick=2, solver=fake, sync=lambda: None, keep_fields=False)
+    srv = Server("float", hbm_budget=2 * one, steps_per_tick=2, solver=fake, timer=FakeTimer(), keep_fields=False)
     results, _ = srv.run(trace)
     assert len(results) == 10
     assert srv.peak_used <= 2 * one
@@ -70,7 +84,7 @@ def test_budget_bounds_the_resident_set():
 
 def test_request_larger_than_budget_is_refused():
     trace = [W.InstanceSpec("cavity", 2048, 0.01, 1e-6, steps=5, arrival=0.0, rid=0)]
-    srv = Server("float", hbm_budget=1 << 20, solver=FakeSolver(), sync=lambda: None, keep_fields=False)
+    srv = Server("float", hbm_budget=1 << 20, solver=FakeSolver(), timer=FakeTimer(), keep_fields=False)
     with pytest.raises(ValueError):
         srv.run(trace)
 

@@ -94,6 +94,9 @@ _SIGS = {
     "xg_kernel_time_isolated": ([c_void_p, c_int32, c_int32, POINTER(c_double)], c_int),
     "xg_api_table": ([], c_void_p),
     "xg_copy_probe": ([c_int64, c_int32, POINTER(c_double), POINTER(c_double)], c_int),
+    "xg_event_record": ([POINTER(c_void_p)], c_int),
+    "xg_event_elapsed": ([c_void_p, c_void_p, POINTER(c_double)], c_int),
+    "xg_event_destroy": ([c_void_p], c_int),
     "xg_batch_create": ([c_void_p, c_int32, POINTER(c_void_p)], c_int),
     "xg_batch_destroy": ([c_void_p], c_int),
     "xg_batch_add": ([c_void_p, c_void_p, c_void_p, c_int64, POINTER(c_void_p), c_int32, POINTER(c_int64)], c_int),

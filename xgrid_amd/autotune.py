@@ -28,6 +28,7 @@ CANDIDATES = [
     (32, 64, 512, 8, "lds"),
     (24, 64, 256, 6, "lds"),
     (40, 64, 512, 6, "reg"),
+    (40, 64, 512, 5, "reg"),
     (24, 64, 512, 4, "reg"),
 ]
 

@@ -1205,6 +1205,34 @@ int xg_kernel_time_isolated(xg_kernel* k, int32_t sid, int32_t reps, double* ms_
 
 const xg_api* xg_api_table(void) { return &g_api; }
 
+int xg_event_record(void** out) {
+  if (int rc = check_ready()) return rc;
+  if (!out) return fail(XG_ERR_ARG, "null output");
+  hipEvent_t e;
+  XG_HIP(hipEventCreate(&e));
+  hipError_t err = hipEventRecord(e, g_dev.stream);
+  if (err != hipSuccess) {
+    (void)hipEventDestroy(e);
+    return fail(XG_ERR_HIP, std::string("hipEventRecord: ") + hipGetErrorString(err));
+  }
+  *out = (void*)e;
+  return XG_OK;
+}
+
+int xg_event_elapsed(void* a, void* b, double* ms) {
+  if (!a || !b || !ms) return fail(XG_ERR_ARG, "null event");
+  XG_HIP(hipEventSynchronize((hipEvent_t)b));
+  float f = 0.f;
+  XG_HIP(hipEventElapsedTime(&f, (hipEvent_t)a, (hipEvent_t)b));
+  *ms = (double)f;
+  return XG_OK;
+}
+
+int xg_event_destroy(void* e) {
+  if (e) (void)hipEventDestroy((hipEvent_t)e);
+  return XG_OK;
+}
+
 int xg_copy_probe(int64_t bytes, int32_t iters, double* gbps, double* ms_per_iter) {
   if (int rc = check_ready()) return rc;
   if (bytes < 32 || iters < 1) return fail(XG_ERR_ARG, "invalid probe size");

@@ -38,6 +38,48 @@ def footprint(spec: InstanceSpec, elem_bytes: int) -> int:
     return grids * spec.n * spec.n * (3 * elem_bytes + 4)
 
 
+class _DeviceTimer:
+    """Device time spent in serving ticks: an event before and after each tick's
+    work on the backend stream (the gaps between ticks are not counted)."""
+
+    def __init__(self):
+        self.lib = _lib.load()
+        self.pending = []
+        self.total_ms = 0.0
+
+    def begin(self):
+        self._a = self._record()
+
+    def end(self):
+        self.pending.append((self._a, self._record()))
+        if len(self.pending) > 64:
+            self._drain(len(self.pending) - 32)
+
+    def sync(self):
+        _lib.check(self.lib.xg_synchronize(), "xg_synchronize")
+
+    def busy_seconds(self) -> float:
+        self._drain(len(self.pending))
+        t, self.total_ms = self.total_ms, 0.0
+        return t / 1e3
+
+    def _record(self):
+        import ctypes
+        e = ctypes.c_void_p()
+        _lib.check(self.lib.xg_event_record(ctypes.byref(e)), "xg_event_record")
+        return e.value
+
+    def _drain(self, n: int):
+        import ctypes
+        for a, b in self.pending[:n]:
+            ms = ctypes.c_double()
+            _lib.check(self.lib.xg_event_elapsed(a, b, ctypes.byref(ms)), "xg_event_elapsed")
+            self.total_ms += ms.value
+            self.lib.xg_event_destroy(a)
+            self.lib.xg_event_destroy(b)
+        self.pending = self.pending[n:]
+
+
 @dataclass
 class Result:
     rid: int
@@ -62,13 +104,13 @@ def _summary(results: list, wall: float, busy: float) -> dict:
 
 class Server:
     def __init__(self, precision: str = "float", hbm_budget: int = 16 << 30, steps_per_tick: int = 8,
-                 graphs: bool = False, keep_fields: bool = True, solver=None, sync=None):
+                 graphs: bool = False, keep_fields: bool = True, solver=None, timer=None):
         self.ms = solver if solver is not None else MultiSolver(precision, graphs=graphs)
         self.elem = 4 if precision == "float" else 8
         self.budget = int(hbm_budget)
         self.steps_per_tick = int(steps_per_tick)
         self.keep_fields = keep_fields
-        self.sync = sync if sync is not None else (lambda: _lib.check(_lib.load().xg_synchronize()))
+        self.timer = timer if timer is not None else _DeviceTimer()
         self.peak_used = 0
         self.peak_active = 0
         self.ticks = 0
@@ -81,7 +123,6 @@ class Server:
         active = {}                      # handle -> [spec, steps done, admitted time]
         used = 0
         results = {}
-        busy = 0.0
         for s in trace:
             if footprint(s, self.elem) > self.budget:
                 raise ValueError(f"request {s.rid} needs more than the whole HBM budget")
@@ -105,10 +146,10 @@ class Server:
                 continue
             k = max(0, min(self.steps_per_tick, min(s.steps - d for s, d, _ in active.values())))
             if k:
-                tb = now()
+                # no wait here: the host prepares the next tick while the device runs this one
+                self.timer.begin()
                 self.ms.step(k)
-                self.sync()
-                busy += now() - tb
+                self.timer.end()
                 self.ticks += 1
             finished = []
             for h, rec in active.items():
@@ -119,13 +160,17 @@ class Server:
                 s, _, adm = active.pop(h)
                 grids = self.ms.remove(h)
                 fields = {}
-                if self.keep_fields:
+                if self.keep_fields:           # the copy waits for the request's last step
                     names = "bpuv" if s.kind == "cavity" else "u"
                     fields = {c: g.now.copy() for c, g in zip(names, grids)}
+                else:
+                    self.timer.sync()
                 del grids
                 used -= footprint(s, self.elem)
                 results[s.rid] = Result(s.rid, s.arrival * time_scale, adm, now(), fields)
+        self.timer.sync()
         wall = now()
+        busy = self.timer.busy_seconds()
         return results, _summary(list(results.values()), wall, busy)
 
     def close(self) -> None:
