@@ -18,44 +18,75 @@ from .precision import precision_scope
 
 
 class MultiSolver:
+    """Cavity and 2-D diffusion instances in batches, one precision.  Instances of
+    a kernel with a blocked loop may be split by grid side into size classes
+    (`size_classes` are the class boundaries), each class a batch of its own with
+    its own tuned tile shape.  On the mixed 64-1024 batch every class picked the
+    same shape and the extra launches cost more than they gained, so by default
+    there is one batch per kernel."""
+
     KINDS = ("cavity", "diffusion")
 
-    def __init__(self, precision: str = "double", graphs: bool = True, max_blocks: int = 0):
+    def __init__(self, precision: str = "double", graphs: bool = True, max_blocks: int = 0,
+                 size_classes: tuple = ()):
+        from . import autotune
         self.precision = precision
         self.ops = {}
+        self.blocked = {}
         with precision_scope(precision):
             for kind, make in (("cavity", W.make_cavity), ("diffusion", W.make_diffusion_2d)):
                 op = make()
                 op.native, op.depth = build_kernel(op)
                 self.ops[kind] = op
+                self.blocked[kind] = bool(op.native.lowered.loops)
         self.graphs, self.max_blocks = graphs, max_blocks
-        self.batches = {k: Batch(op, graphs=graphs, max_blocks=max_blocks) for k, op in self.ops.items()}
-        self.members = {}            # handle -> (kind, member id, grids, spec)
+        self.size_classes = tuple(size_classes)
+        self.natives = {}            # batch key -> the build its batch runs
+        self.batches = {}            # batch key -> Batch (created on first use)
+        self.members = {}            # handle -> (batch key, member id, grids, spec)
         self._next = 1
-        self.shapes = {k: op.native.block_shape for k, op in self.ops.items()}
         self.tuning = {}
 
+    def key(self, spec: W.InstanceSpec) -> str:
+        if not self.blocked[spec.kind] or not self.size_classes:
+            return spec.kind
+        c = sum(1 for b in self.size_classes if spec.n >= b)
+        return f"{spec.kind}/{c}"
+
+    @property
+    def shapes(self) -> dict:
+        return {k: b.native.block_shape for k, b in self.batches.items()}
+
+    def _batch(self, key: str) -> Batch:
+        b = self.batches.get(key)
+        if b is None:
+            op = self.ops[key.split("/")[0]]
+            b = Batch(op, graphs=self.graphs, max_blocks=self.max_blocks, native=self.natives.get(key))
+            self.batches[key] = b
+        return b
+
     def autotune(self, specs: list, limit: int = 48, use_cache: bool = True) -> dict:
-        """Pick each kernel's blocked-loop tile shape on fresh copies of a sample of
-        `specs` (before any instance is added) and rebuild its batch with it."""
+        """Pick the blocked-loop tile shape of each batch (kernel and size class) on
+        fresh copies of a sample of its `specs`, before any instance is added."""
         from . import autotune
         if self.members:
             raise RuntimeError("autotune before adding instances")
-        for kind, op in self.ops.items():
-            mine = [s for s in specs if s.kind == kind]
-            if not mine:
+        groups = {}
+        for s in specs:
+            groups.setdefault(self.key(s), []).append(s)
+        for key, mine in groups.items():
+            kind = key.split("/")[0]
+            if not self.blocked[kind]:
                 continue
+            op = self.ops[kind]
             with precision_scope(self.precision):
-                if not autotune.has_blocked_loop(op):
-                    continue
                 res = autotune.tune_batch(op, lambda s: W.instance_grids(s, Grid),
                                           autotune.sample(mine, limit, size=lambda s: s.n), self.precision,
                                           max_blocks=self.max_blocks, use_cache=use_cache)
-                native = HipKernel(op, res["shape"])
-            self.batches[kind].close()
-            self.batches[kind] = Batch(op, graphs=self.graphs, max_blocks=self.max_blocks, native=native)
-            self.shapes[kind] = res["shape"]
-            self.tuning[kind] = res
+                self.natives[key] = HipKernel(op, res["shape"])
+            if key in self.batches:
+                self.batches.pop(key).close()
+            self.tuning[key] = res
         return self.tuning
 
     def grids_for(self, spec: W.InstanceSpec):
@@ -64,16 +95,17 @@ class MultiSolver:
 
     def add(self, spec: W.InstanceSpec) -> int:
         grids, scalars = self.grids_for(spec)
-        mid = self.batches[spec.kind].add(*grids, *scalars)
+        key = self.key(spec)
+        mid = self._batch(key).add(*grids, *scalars)
         h = self._next
         self._next += 1
-        self.members[h] = (spec.kind, mid, grids, spec)
+        self.members[h] = (key, mid, grids, spec)
         return h
 
     def remove(self, handle: int) -> list:
         """Drop an instance; returns its grids (still valid, holding its fields)."""
-        kind, mid, grids, _ = self.members.pop(handle)
-        self.batches[kind].remove(mid)
+        key, mid, grids, _ = self.members.pop(handle)
+        self.batches[key].remove(mid)
         return grids
 
     def grids(self, handle: int) -> list:
@@ -81,9 +113,11 @@ class MultiSolver:
 
     def step(self, steps: int = 1) -> None:
         for b in self.batches.values():
-            b.step(steps)
+            if len(b):
+                b.step(steps)
 
     def set_options(self, graphs: bool = True, max_blocks: int = 0) -> None:
+        self.graphs, self.max_blocks = graphs, max_blocks
         for b in self.batches.values():
             b.set_options(graphs, max_blocks)
 
@@ -97,4 +131,5 @@ class MultiSolver:
     def close(self) -> None:
         for b in self.batches.values():
             b.close()
+        self.batches.clear()
         self.members.clear()
