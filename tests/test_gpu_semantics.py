@@ -421,3 +421,50 @@ def test_register_layout_declines_when_halo_too_small(double):
     run_pair(creep2, (du, dv), (hu, hv), steps=2)
     assert creep2.native.loop_info(0)["state"] == -1
     assert_bitexact(du, hu)
+
+
+def test_inline_c_and_external_function_in_host_code(double):
+    """`with xgrid.c()` text and an external C function run in the host stub,
+    between launches, as in the reference's generated C; the stencil statement
+    still runs on the device."""
+    from xgrid_amd.types import Integer
+
+    @xgrid.external(name="abs", includes=["stdlib.h"], typecheck_override=lambda args: Integer(4))
+    def c_abs(x: int) -> int:
+        ...
+
+    @xgrid.kernel()
+    def k(u: f1, a: int) -> int:
+        b = c_abs(a)
+        with xgrid.c():
+            "b = b * 2 + 1;"
+        u[0] = u[0] + 1.0
+        return b
+
+    d, h = grid_pair(xgrid, (33,), float, None, np.arange(33.0))
+    ck = CpuKernel(k)
+    for a in (-5, 7, 0):
+        assert k(d, a) == ck(h, a) == abs(a) * 2 + 1
+    assert_bitexact(d, h)
+    assert k.native.lowered.graphs_ok is False        # host side effects: no graph replay
+
+
+def test_bool_grid_statements(double):
+    """A bool grid written from comparisons and read back as a condition."""
+    b2 = xgrid.grid[bool, 2]
+
+    @xgrid.kernel()
+    def k(u: f2, flag: b2, lim: float) -> None:
+        flag[0, 0] = u[0, 0] > lim and u[0, 1] <= lim
+        u[0, 0] = u[0, 0] * 0.5 if flag[0, 0][0] else u[0, 0] + 1.0
+
+    rng = np.random.default_rng(3)
+    u0 = rng.random((20, 31))
+    mask = np.zeros((20, 31), np.int32)
+    mask[:, -1] = 1
+    du, hu = grid_pair(xgrid, (20, 31), float, mask, u0)
+    df, hf = grid_pair(xgrid, (20, 31), bool, mask)
+    run_pair(k, (du, df, 0.5), (hu, hf, 0.5), steps=3)
+    assert df.now.dtype == np.bool_
+    assert_bitexact(du, hu)
+    assert np.array_equal(df.now[:, :-1], hf.now[:, :-1])

@@ -189,6 +189,10 @@ class _ExprC:
 
 
 def function_symbol(op) -> str:
+    """C name of a user function: an external keeps its own name (its header
+    declares it, as in the reference's generated C); others get a prefix."""
+    if op.mode == "external":
+        return op.name
     return "xgf_" + op.c_name
 
 
@@ -965,7 +969,8 @@ class Emitter:
             fi = op.ir
             args = ", ".join(f"{ctype_name(t)} {n}" for n, t in fi.args)
             decl = f"{ctype_name(fi.ret)} {function_symbol(op)}({args})"
-            out.append(("extern " if op.mode == "external" else "static ") + decl + ";")
+            if op.mode != "external":      # externals are declared by their includes
+                out.append("static " + decl + ";")
         for op in self.functions:
             if op.mode == "external":
                 continue
