@@ -49,6 +49,26 @@ template <> struct Tile<1> { static constexpr int E0 = 1024, E1 = 1, E2 = 1, NC 
 template <> struct Tile<2> { static constexpr int E0 = 4, E1 = 64, E2 = 1, NC = 1, CH = 1; };
 template <> struct Tile<3> { static constexpr int E0 = 4, E1 = 4, E2 = 64, NC = 4, CH = XG_TILE3_CHUNKS; };
 
+/* Every pointer in a launch header, view or batch table is device memory from
+ * hipMalloc (global address space).  A pointer read out of memory (a batch
+ * table entry) is generic to the compiler, which then emits flat accesses:
+ * those wait on the LDS counter too and serialize against LDS traffic.  This
+ * cast states the address space, so accesses are global loads and stores. */
+template <class T>
+using gptr = __attribute__((address_space(1))) T*;
+template <class T>
+__device__ __forceinline__ gptr<T> gp(const void* p) {
+  return (gptr<T>)(p);
+}
+/* The views and parameter blocks a batch table points to are written by the
+ * host before the launch and only read by kernels: reading them through the
+ * constant address space lets uniform reads be scalar loads into SGPRs, as
+ * kernel arguments are. */
+template <class T>
+__device__ __forceinline__ const __attribute__((address_space(4))) T* cp(const void* p) {
+  return (const __attribute__((address_space(4))) T*)(p);
+}
+
 /* The hardware hands workgroup b to XCD (b mod 8).  Give each XCD one
  * contiguous run of tiles so vertical neighbours stay in that XCD's L2. */
 __device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t nb) {
@@ -105,28 +125,28 @@ __device__ __forceinline__ int64_t off(const int32_t* shape, int o0, int o1, int
 template <class T, int D, int OV, bool SAFE>
 __device__ __forceinline__ T ld(const xg_gview& g, const int32_t* sh, int64_t xc, int o0, int o1, int o2,
                                 int i0, int i1, int i2) {
-  if (SAFE) return static_cast<const T*>(g.ptr)[xc + off<D>(sh, o0, o1, o2)];
+  if (SAFE) return gp<const T>(g.ptr)[xc + off<D>(sh, o0, o1, o2)];
   i0 += o0;
   i1 += o1;
   i2 += o2;
   int64_t k = index<D, OV>(g, i0, i1, i2);
   const int64_t n = cells<D>(g.shape);
   k = k < 0 ? 0 : (k >= n ? n - 1 : k);
-  return static_cast<const T*>(g.ptr)[k];
+  return gp<const T>(g.ptr)[k];
 }
 
 /* Store one cell; a store outside the array is dropped. */
 template <class T, int D, int OV>
 __device__ __forceinline__ void st(const xg_gview& g, int i0, int i1, int i2, const T& v) {
   const int64_t k = index<D, OV>(g, i0, i1, i2);
-  if (k >= 0 && k < cells<D>(g.shape)) static_cast<T*>(g.ptr)[k] = v;
+  if (k >= 0 && k < cells<D>(g.shape)) gp<T>(g.ptr)[k] = v;
 }
 
 template <class S, int OV>
 __device__ __forceinline__ void store(const xg_launch_hdr& H, int i0, int i1, int i2, const typename S::T& v) {
   typedef typename S::T T;
   if (H.dom.flags & XG_F_TO_TMP) {
-    static_cast<T*>(H.src.ptr)[lin<S::D>(H.out.shape, i0, i1, i2)] = v;
+    gp<T>(H.src.ptr)[lin<S::D>(H.out.shape, i0, i1, i2)] = v;
   } else {
     st<T, S::D, OV>(H.out, i0 + S::SO0, i1 + S::SO1, i2 + S::SO2, v);
   }
@@ -242,7 +262,7 @@ __device__ __forceinline__ void box_cells(const xg_launch_hdr& H, const typename
     if (MASKED) {
 #pragma unroll
       for (int n = 0; n < NC; ++n)
-        tc.in[n] = tc.in[n] && H.dom.mask[lin<D>(H.out.shape, tc.c[n][0], tc.c[n][1], tc.c[n][2])] == H.dom.klass;
+        tc.in[n] = tc.in[n] && gp<const int32_t>(H.dom.mask)[lin<D>(H.out.shape, tc.c[n][0], tc.c[n][1], tc.c[n][2])] == H.dom.klass;
     }
     T v[NC];
     if (tile_safe<S>(H, tc.tlo, tc.thi)) {
@@ -269,7 +289,7 @@ __device__ void run_main(const xg_launch_hdr& H, const typename S::Refs& R, cons
     const int64_t stride = ntiles * blockDim.x;
     for (int64_t k = (int64_t)tile * blockDim.x + threadIdx.x; k < dom.count; k += stride) {
       int i0, i1, i2;
-      unlin<D>(H.out.shape, dom.list[k], i0, i1, i2);
+      unlin<D>(H.out.shape, gp<const int32_t>(dom.list)[k], i0, i1, i2);
       apply<S, OV>(H, R, P, i0, i1, i2);
     }
   } else if (dom.kind == XG_DOM_MASKED) {
@@ -289,8 +309,8 @@ __device__ void run_pingpong(const xg_launch_hdr& H, const typename S::Refs& R, 
   const xg_dom& dom = H.dom;
   const int32_t zero[3] = {0, 0, 0};
   const int32_t* shape = H.out.shape;
-  const T* __restrict__ src = static_cast<const T*>(H.src.ptr);
-  T* __restrict__ out = static_cast<T*>(H.out.ptr);
+  const gptr<const T> __restrict__ src = gp<const T>(H.src.ptr);
+  const gptr<T> __restrict__ out = gp<T>(H.out.ptr);
   for (int ch = 0; ch < Tile<D>::CH; ++ch) {
   TileCells<D> tc(zero, shape, tile, ch);
   if (tc.tlo[0] >= shape[0]) break;
@@ -308,7 +328,7 @@ __device__ void run_pingpong(const xg_launch_hdr& H, const typename S::Refs& R, 
       if (D > 2) m = m && c[2] >= dom.lo[2] && c[2] < dom.hi[2];
       member[n] = m;
     } else {
-      member[n] = dom.mask[lc[n]] == dom.klass;
+      member[n] = gp<const int32_t>(dom.mask)[lc[n]] == dom.klass;
     }
   }
   T v[NC];
@@ -347,7 +367,7 @@ __device__ __forceinline__ void for_tile(const int32_t* lo, const int32_t* hi, u
 template <class T, int D, int OV>
 __device__ void run_phase2(const xg_launch_hdr& H, uint32_t tile, int64_t ntiles) {
   const xg_dom& dom = H.dom;
-  const T* tmp = static_cast<const T*>(H.src.ptr);
+  const gptr<const T> tmp = gp<const T>(H.src.ptr);
   const int32_t* shape = H.out.shape;
   auto f = [&](int i0, int i1, int i2) {
     st<T, D, OV>(H.out, i0 + dom.so[0], i1 + dom.so[1], i2 + dom.so[2], tmp[lin<D>(shape, i0, i1, i2)]);
@@ -356,12 +376,12 @@ __device__ void run_phase2(const xg_launch_hdr& H, uint32_t tile, int64_t ntiles
     const int64_t stride = ntiles * blockDim.x;
     for (int64_t k = (int64_t)tile * blockDim.x + threadIdx.x; k < dom.count; k += stride) {
       int i0, i1, i2;
-      unlin<D>(shape, dom.list[k], i0, i1, i2);
+      unlin<D>(shape, gp<const int32_t>(dom.list)[k], i0, i1, i2);
       f(i0, i1, i2);
     }
   } else if (dom.kind == XG_DOM_MASKED) {
     for_tile<D>(dom.lo, dom.hi, tile, [&](int i0, int i1, int i2) {
-      if (dom.mask[lin<D>(shape, i0, i1, i2)] == dom.klass) f(i0, i1, i2);
+      if (gp<const int32_t>(dom.mask)[lin<D>(shape, i0, i1, i2)] == dom.klass) f(i0, i1, i2);
     });
   } else {
     for_tile<D>(dom.lo, dom.hi, tile, f);
@@ -378,7 +398,7 @@ __device__ void run_block(const xg_launch_hdr& H, const typename S::Refs& R, con
   if (dom.kind == XG_DOM_LIST) {
     for (int64_t k = threadIdx.x; k < dom.count; k += blockDim.x) {
       int i0, i1, i2;
-      unlin<D>(H.out.shape, dom.list[k], i0, i1, i2);
+      unlin<D>(H.out.shape, gp<const int32_t>(dom.list)[k], i0, i1, i2);
       apply<S, OV>(H, R, P, i0, i1, i2);
     }
     return;
@@ -389,7 +409,7 @@ __device__ void run_block(const xg_launch_hdr& H, const typename S::Refs& R, con
     int j0, j1, j2;
     unlin<3>(ext, k, j0, j1, j2);
     const int i0 = dom.lo[0] + j0, i1 = (D > 1 ? dom.lo[1] : 0) + j1, i2 = (D > 2 ? dom.lo[2] : 0) + j2;
-    if (dom.kind == XG_DOM_MASKED && dom.mask[lin<D>(H.out.shape, i0, i1, i2)] != dom.klass) continue;
+    if (dom.kind == XG_DOM_MASKED && gp<const int32_t>(dom.mask)[lin<D>(H.out.shape, i0, i1, i2)] != dom.klass) continue;
     apply<S, OV>(H, R, P, i0, i1, i2);
   }
 }

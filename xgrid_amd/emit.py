@@ -409,17 +409,21 @@ class Emitter:
         out.append(f'extern "C" __global__ void __launch_bounds__(256) xg_k{sid}({args}) '
                    f"{{ xg::run_main<xg_s{sid}, XG_OVERSTEP>(H, R, P, {remap}, gridDim.x); }}")
         # batched: one launch over the tiles of many instances (entries in a device table)
+        # The entry's header, views and parameters are copied into locals first: read
+        # through the table's generic pointers they could alias the kernel's own
+        # stores, and would be re-read from memory for every cell.
         bargs = "const xg_bent* __restrict__ E, const int32_t n, const int64_t total"
-        cast = f"*static_cast<const xg_r{sid}*>(e.refs), *static_cast<const xg_p{sid}*>(e.params)"
+        local = (f"const xg_launch_hdr h = e.hdr; const xg_r{sid} r = *xg::cp<xg_r{sid}>(e.refs); "
+                 f"const xg_p{sid} p = *xg::cp<xg_p{sid}>(e.params);")
         out.append(f'extern "C" __global__ void __launch_bounds__(256) xg_k{sid}_b({bargs}) '
                    f"{{ xg::for_batch_tiles(E, n, total, [&](const xg_bent& e, uint32_t t) "
-                   f"{{ xg::run_main<xg_s{sid}, XG_OVERSTEP>(e.hdr, {cast}, t, e.ntiles); }}); }}")
+                   f"{{ {local} xg::run_main<xg_s{sid}, XG_OVERSTEP>(h, r, p, t, e.ntiles); }}); }}")
         if st.mode == "pingpong":
             out.append(f'extern "C" __global__ void __launch_bounds__(256) xg_k{sid}_pp({args}) '
                        f"{{ xg::run_pingpong<xg_s{sid}, XG_OVERSTEP>(H, R, P, {remap}); }}")
             out.append(f'extern "C" __global__ void __launch_bounds__(256) xg_k{sid}_ppb({bargs}) '
                        f"{{ xg::for_batch_tiles(E, n, total, [&](const xg_bent& e, uint32_t t) "
-                       f"{{ xg::run_pingpong<xg_s{sid}, XG_OVERSTEP>(e.hdr, {cast}, t); }}); }}")
+                       f"{{ {local} xg::run_pingpong<xg_s{sid}, XG_OVERSTEP>(h, r, p, t); }}); }}")
 
     def _device_function(self, op, out: list):
         f = op.ir
@@ -476,7 +480,7 @@ class Emitter:
         o("  const int nt1 = (N1 + C1 - 1) / C1;")
         o("  const int g0 = (tile / nt1) * C0, g1 = (tile % nt1) * C1;")
         o("  const int r0 = g0 - HL, r1 = g1 - HL;")
-        o("  const T* __restrict__ src = static_cast<const T*>(TH.src.ptr);")
+        o("  const xg::gptr<const T> __restrict__ src = xg::gp<const T>(TH.src.ptr);")
         o("  uint32_t any = 0;")
         o("#pragma unroll")
         o("  for (int n = 0; n < NC; ++n) {")
@@ -488,7 +492,7 @@ class Emitter:
         o("      buf[idx] = in ? src[c] : T(0);")
         o("      uint32_t mb = 0u;")
         o("      if (in) {")
-        o("        const int32_t k = TH.mask ? TH.mask[c] : 0;")
+        o("        const int32_t k = TH.mask ? xg::gp<const int32_t>(TH.mask)[c] : 0;")
         o("        mb = " + " | ".join(f"(k == TH.klass[{j}] ? {1 << j}u : 0u)" for j in range(m)) + ";")
         o("      }")
         o("      mem[idx] = (uint8_t)mb;")
@@ -502,7 +506,7 @@ class Emitter:
         o("  }")
         o("  any = xg::block_or(any);")
         o("  const int ns = scount < SL ? scount : SL;")
-        o("  if (CHECK && threadIdx.x == 0 && scount > SL) *TH.fail = 1;   /* list overflow: decline */")
+        o("  if (CHECK && threadIdx.x == 0 && scount > SL) *xg::gp<int32_t>(TH.fail) = 1;   /* list overflow: decline */")
         o(f"  const bool safe = TH.uniform && r0 - {rl[0]} >= 0 && r0 + RHH + {rh[0]} <= N0 && "
           f"r1 - {rl[1]} >= 0 && r1 + RWW + {rh[1]} <= N1;")
         # Cells at least `reach` inside the region are computed; their reads stay in the
@@ -604,7 +608,7 @@ class Emitter:
         o("  if (safe) iterate(xg::bconst<true>()); else iterate(xg::bconst<false>());")
         o("  const T* lds = buf + cb;")
         o("  const uint8_t* vld = vl + (CHECK ? cb : 0);")
-        o("  T* __restrict__ dst = static_cast<T*>(TH.dst.ptr);")
+        o("  const xg::gptr<T> __restrict__ dst = xg::gp<T>(TH.dst.ptr);")
         o("  constexpr int NCORE = (C0 * C1 + TBT - 1) / TBT;")
         o("#pragma unroll")
         o("  for (int n = 0; n < NCORE; ++n) {")
@@ -613,26 +617,46 @@ class Emitter:
         o("    if (q < C0 * C1 && i0 < N0 && i1 < N1) {")
         o("      const int li = (HL + q / C1) * RWW + HL + q % C1;")
         o("      dst[(int64_t)i0 * N1 + i1] = lds[li];")
-        o("      if (CHECK && !vld[li]) *TH.fail = 1;")
+        o("      if (CHECK && !vld[li]) *xg::gp<int32_t>(TH.fail) = 1;")
         o("    }")
         o("  }")
         o("}")
         kargs = ["const xg_tb_hdr TH"] + [f"const xg_r{s.stmt.sid} R{j}, const xg_p{s.stmt.sid} P{j}"
                                           for j, s in enumerate(L.stmts)]
         call = ", ".join(["TH"] + [f"R{j}, P{j}" for j in range(m)] + ["(int)xg::xcd_remap(blockIdx.x, gridDim.x)"])
-        o(f'extern "C" __global__ void __launch_bounds__({L.threads}) {L.kernel}({", ".join(kargs)}) '
+        o(f'extern "C" __global__ void {self._tb_bounds(L)} {L.kernel}({", ".join(kargs)}) '
           f"{{ {L.kernel}_body<false>({call}); }}")
-        o(f'extern "C" __global__ void __launch_bounds__({L.threads}) {L.kernel}_chk({", ".join(kargs)}) '
+        o(f'extern "C" __global__ void {self._tb_bounds(L, True)} {L.kernel}_chk({", ".join(kargs)}) '
           f"{{ {L.kernel}_body<true>({call}); }}")
-        # batched: the tiles of many instances; the barrier lets a workgroup reuse its LDS for the next tile
-        bcall = ", ".join(["e.th"] + [f"*static_cast<const xg_r{s.stmt.sid}*>(e.refs[{j}]), "
-                                      f"*static_cast<const xg_p{s.stmt.sid}*>(e.params[{j}])"
-                                      for j, s in enumerate(L.stmts)] + ["(int)t"])
+        self._tb_batched(L, out)
+
+    @staticmethod
+    def _tb_bounds(L: LoopMeta, check: bool = False) -> str:
+        """Launch bounds of a blocked kernel: two workgroups per CU (its LDS or
+        registers allow no more), so each SIMD must hold threads / 128 waves; the
+        attribute keeps the compiler's register budget at that occupancy.  The
+        checking variant runs once per plan and needs more LDS: no target."""
+        if check:
+            return f"__launch_bounds__({L.threads})"
+        return f"__launch_bounds__({L.threads}) __attribute__((amdgpu_waves_per_eu({max(1, L.threads // 128)})))"
+
+    def _tb_batched(self, L: LoopMeta, out: list):
+        """Batched variants of a blocked loop's kernel: the tiles of many instances.
+        Each entry's header, views and parameters are copied into locals before the
+        body runs (read through the table's generic pointers they could alias the
+        body's stores and would be re-read for every cell); the barrier lets the
+        workgroup reuse its LDS for its next tile."""
+        loc = ["const xg_tb_hdr th = e.th;"]
+        for j, s in enumerate(L.stmts):
+            sid = s.stmt.sid
+            loc.append(f"const xg_r{sid} r{j} = *xg::cp<xg_r{sid}>(e.refs[{j}]);")
+            loc.append(f"const xg_p{sid} p{j} = *xg::cp<xg_p{sid}>(e.params[{j}]);")
+        bcall = ", ".join(["th"] + [f"r{j}, p{j}" for j in range(len(L.stmts))] + ["(int)t"])
         for suffix, chk in (("_b", "false"), ("_chkb", "true")):
-            o(f'extern "C" __global__ void __launch_bounds__({L.threads}) {L.kernel}{suffix}('
-              f"const xg_btent* __restrict__ E, const int32_t n, const int64_t total) "
-              f"{{ xg::for_batch_tiles(E, n, total, [&](const xg_btent& e, uint32_t t) "
-              f"{{ {L.kernel}_body<{chk}>({bcall}); __syncthreads(); }}); }}")
+            out.append(f'extern "C" __global__ void {self._tb_bounds(L, chk == "true")} {L.kernel}{suffix}('
+                       f"const xg_btent* __restrict__ E, const int32_t n, const int64_t total) "
+                       f"{{ xg::for_batch_tiles(E, n, total, [&](const xg_btent& e, uint32_t t) "
+                       f"{{ {' '.join(loc)} {L.kernel}_body<{chk}>({bcall}); __syncthreads(); }}); }}")
 
     def _tb_kernel_reg(self, L: LoopMeta, out: list):
         """Blocked kernel of one loop.  A workgroup owns a core tile of the loop grid
@@ -678,7 +702,7 @@ class Emitter:
         o("  const int g0 = (tile / nt1) * C0, g1 = (tile % nt1) * C1;")
         o("  const int r0 = g0 - HL, cg = g1 - HL + lane;   /* region origin row; this lane's grid column */")
         o("  const int br = w * BR;                          /* the band's first region row */")
-        o("  const T* __restrict__ src = static_cast<const T*>(TH.src.ptr);")
+        o("  const xg::gptr<const T> __restrict__ src = xg::gp<const T>(TH.src.ptr);")
         o("  T V[NQ];")
         o("  uint8_t OK[CHECK ? NQ : 1];")
         o("  uint32_t MB[(BR + 3) / 4];          /* statement-membership bits: 8 per row */")
@@ -695,7 +719,7 @@ class Emitter:
         o("    if (q >= RE && q < RE + BR) {")
         o("      uint32_t mb = 0u;")
         o("      if (in) {")
-        o("        const int32_t k = TH.mask ? TH.mask[c] : 0;")
+        o("        const int32_t k = TH.mask ? xg::gp<const int32_t>(TH.mask)[c] : 0;")
         o("        mb = " + " | ".join(f"(k == TH.klass[{j}] ? {1 << j}u : 0u)" for j in range(m)) + ";")
         o("      }")
         o("      MB[(q - RE) >> 2] |= mb << (8 * ((q - RE) & 3));")
@@ -787,32 +811,24 @@ class Emitter:
         o("  }")
         o("  };")
         o("  if (safe) iterate(xg::bconst<true>()); else iterate(xg::bconst<false>());")
-        o("  T* __restrict__ dst = static_cast<T*>(TH.dst.ptr);")
+        o("  const xg::gptr<T> __restrict__ dst = xg::gp<T>(TH.dst.ptr);")
         o("#pragma unroll")
         o("  for (int q = 0; q < BR; ++q) {")
         o("    const int lr = br + q, gr = r0 + lr;")
         o("    if (lr >= HL && lr < HL + C0 && lane >= HL && lane < HL + C1 && gr < N0 && cg < N1) {")
         o("      dst[(int64_t)gr * N1 + cg] = V[q + RE];")
-        o("      if (CHECK && !OK[q + RE]) *TH.fail = 1;")
+        o("      if (CHECK && !OK[q + RE]) *xg::gp<int32_t>(TH.fail) = 1;")
         o("    }")
         o("  }")
         o("}")
         kargs = ["const xg_tb_hdr TH"] + [f"const xg_r{s_.stmt.sid} R{j}, const xg_p{s_.stmt.sid} P{j}"
                                           for j, s_ in enumerate(L.stmts)]
         call = ", ".join(["TH"] + [f"R{j}, P{j}" for j in range(m)] + ["(int)xg::xcd_remap(blockIdx.x, gridDim.x)"])
-        o(f'extern "C" __global__ void __launch_bounds__({L.threads}) {L.kernel}({", ".join(kargs)}) '
+        o(f'extern "C" __global__ void {self._tb_bounds(L)} {L.kernel}({", ".join(kargs)}) '
           f"{{ {L.kernel}_body<false>({call}); }}")
-        o(f'extern "C" __global__ void __launch_bounds__({L.threads}) {L.kernel}_chk({", ".join(kargs)}) '
+        o(f'extern "C" __global__ void {self._tb_bounds(L, True)} {L.kernel}_chk({", ".join(kargs)}) '
           f"{{ {L.kernel}_body<true>({call}); }}")
-        # batched: the tiles of many instances; the barrier lets a workgroup reuse its LDS for the next tile
-        bcall = ", ".join(["e.th"] + [f"*static_cast<const xg_r{s_.stmt.sid}*>(e.refs[{j}]), "
-                                      f"*static_cast<const xg_p{s_.stmt.sid}*>(e.params[{j}])"
-                                      for j, s_ in enumerate(L.stmts)] + ["(int)t"])
-        for suffix, chk in (("_b", "false"), ("_chkb", "true")):
-            o(f'extern "C" __global__ void __launch_bounds__({L.threads}) {L.kernel}{suffix}('
-              f"const xg_btent* __restrict__ E, const int32_t n, const int64_t total) "
-              f"{{ xg::for_batch_tiles(E, n, total, [&](const xg_btent& e, uint32_t t) "
-              f"{{ {L.kernel}_body<{chk}>({bcall}); __syncthreads(); }}); }}")
+        self._tb_batched(L, out)
 
     def device_source(self, stmt_meta: list, groups: list, loops: list = ()) -> str:
         out = ["// generated by xgrid_amd for the MI355X backend", '#include "xgrid_device.h"',
@@ -845,7 +861,7 @@ class Emitter:
             out.append(f'extern "C" __global__ void __launch_bounds__(256) {m.phase2}_b('
                        f"const xg_bent* __restrict__ E, const int32_t n, const int64_t total) "
                        f"{{ xg::for_batch_tiles(E, n, total, [&](const xg_bent& e, uint32_t t) "
-                       f"{{ {ph2}(e.hdr, t, e.ntiles); }}); }}")
+                       f"{{ const xg_launch_hdr h = e.hdr; {ph2}(h, t, e.ntiles); }}); }}")
         for g in groups:
             params = ["const uint32_t en"]
             body = []
@@ -867,9 +883,10 @@ class Emitter:
                 m = stmt_meta[g.first + pos]
                 sid = m.stmt.sid
                 if m.stmt.mode == "inplace":
-                    out.append(f"  if (I[{pos}].hdr.dom.kind != XG_DOM_EMPTY) {{ xg::run_block<xg_s{sid}, XG_OVERSTEP>("
-                               f"I[{pos}].hdr, *static_cast<const xg_r{sid}*>(I[{pos}].refs), "
-                               f"*static_cast<const xg_p{sid}*>(I[{pos}].params)); __syncthreads(); }}")
+                    out.append(f"  if (I[{pos}].hdr.dom.kind != XG_DOM_EMPTY) {{ const xg_launch_hdr h = I[{pos}].hdr; "
+                               f"const xg_r{sid} r = *xg::cp<xg_r{sid}>(I[{pos}].refs); "
+                               f"const xg_p{sid} p = *xg::cp<xg_p{sid}>(I[{pos}].params); "
+                               f"xg::run_block<xg_s{sid}, XG_OVERSTEP>(h, r, p); __syncthreads(); }}")
             out.append("}")
         return "\n".join(out) + "\n"
 
