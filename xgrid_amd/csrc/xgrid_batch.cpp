@@ -288,7 +288,7 @@ int run_group(xg_batch* b, int32_t gi, const std::vector<Pos>& pos, size_t first
       xg_kernel* k = b->inst[(size_t)idx[t]].k;
       StmtPlan& sp = k->stmts[(size_t)pos[first + j].sid];
       const ClassPlan& p = k->grids[(size_t)sp.d.store_arg]->plan(sp.d.klass);
-      ok[j] = effective_mode(k, sp) == XG_MODE_INPLACE && p.count <= kFuseMaxCells;
+      ok[j] = effective_mode(k, sp) == XG_MODE_INPLACE && p.count <= g_dev.fuse_max_cells;
     }
   size_t j = 0;
   while (j < n) {

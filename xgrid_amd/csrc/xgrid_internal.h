@@ -36,6 +36,7 @@ struct Device {
   int cus = 0;
   int64_t hbm = 0;
   char name[256] = {0};
+  int64_t fuse_max_cells = 0;   // a chained statement walks at most this many cells
 };
 extern Device g_dev;
 

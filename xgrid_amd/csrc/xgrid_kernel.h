@@ -14,7 +14,7 @@ namespace xgi {
 
 constexpr int kThreads = 256;
 constexpr int kGroupThreads = 1024;
-constexpr int64_t kFuseMaxCells = 1 << 16;  // a chained statement walks at most this many cells
+constexpr int64_t kFuseMaxCells = 4096;  // default of g_dev.fuse_max_cells (XGRID_FUSE_MAX_CELLS)
 constexpr int64_t kGraphMinLaunches = 4;    // a call with fewer launches runs them directly (cheaper)
 
 // kRecord: the stub's statements are recorded for a batched launch instead of run

@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -329,7 +330,7 @@ int flush(xg_kernel* k) {
     StmtPlan& sp = k->stmts[(size_t)run[i].sid];
     xg_grid* G = k->grids[(size_t)sp.d.store_arg];
     const ClassPlan& p = G->plan(sp.d.klass);
-    ok[i] = effective_mode(k, sp) == XG_MODE_INPLACE && p.count <= kFuseMaxCells;
+    ok[i] = effective_mode(k, sp) == XG_MODE_INPLACE && p.count <= g_dev.fuse_max_cells;
   }
   size_t i = 0;
   while (i < run.size() && rc == XG_OK) {
@@ -751,6 +752,8 @@ int xg_init(int device) {
   g_dev.cus = prop.multiProcessorCount;
   g_dev.hbm = (int64_t)prop.totalGlobalMem;
   std::snprintf(g_dev.name, sizeof g_dev.name, "%s (%s)", prop.name, prop.gcnArchName);
+  g_dev.fuse_max_cells = kFuseMaxCells;
+  if (const char* v = std::getenv("XGRID_FUSE_MAX_CELLS")) g_dev.fuse_max_cells = std::atoll(v);
   g_dev.ready = true;
   return XG_OK;
 }
