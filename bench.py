@@ -281,7 +281,13 @@ def leg_batching(args) -> dict:
     lib = _lib.load()
     specs = W.batch_specs(256, seed=1)
     K = args.batch_steps
-    warm = 2
+    # Both sides are timed in steady state and take the same number of steps (the
+    # results are compared).  A batch's time-level state repeats with a period of
+    # up to 6 steps and each state's graph is captured on its second sighting: 13
+    # warm-up steps reach replay.  Each plain instance is warmed with its 13 calls
+    # in a row, which also keeps the plain path's graphs on (they switch off after
+    # 64 calls without a single repeat).
+    warm = 13
     cells = sum(s.n * s.n for s in specs)
     out = {"workload": f"{len(specs)} instances ({sum(s.kind == 'cavity' for s in specs)} cavity, "
                        f"{sum(s.kind == 'diffusion' for s in specs)} 2-D diffusion), side log-uniform [64, 1024], "

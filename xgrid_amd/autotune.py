@@ -86,11 +86,14 @@ def sample(items: list, limit: int, size=lambda x: x) -> list:
 
 
 def tune_batch(op, make_instance, instances: list, precision: str, candidates: list | None = None,
-               warm: int = 4, steps: int = 4, max_blocks: int = 0, use_cache: bool = True) -> dict:
+               warm: int = 13, steps: int = 6, max_blocks: int = 0, use_cache: bool = True) -> dict:
     """Pick the fastest blocked-loop shape for batches of `op`.
 
     `make_instance(item)` returns (grids, scalars) for a fresh copy of an instance;
-    `instances` are the items to time (a sample of the real batch).  Returns
+    `instances` are the items to time (a sample of the real batch).  The batch's
+    time-level state repeats with a period of up to 6 steps and a state's graph is
+    captured on its second sighting, so `warm` >= 13 steps reach steady replay and
+    `steps` = 6 times one whole period.  Returns
     {"shape": ..., "timings": {shape: ms per step}, "cached": bool}."""
     from .batch import Batch
     candidates = [tuple(c) for c in (candidates or CANDIDATES)]

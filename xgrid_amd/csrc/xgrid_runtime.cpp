@@ -1083,9 +1083,13 @@ int xg_kernel_set_options(xg_kernel* k, int32_t graphs, int32_t fuse, int32_t bl
   k->graphs = graphs != 0;
   k->fuse = fuse != 0;
   k->blocked = blocked != 0;
-  // captured graphs embody the old options
+  // captured graphs embody the old options; setting the options also ends an
+  // automatic switch-off (the caller asked for graphs again)
   for (auto& e : k->cache) destroy_entry(e);
   k->cache.clear();
+  k->graphs_auto_off = false;
+  k->graph_hits = 0;
+  k->graph_misses = 0;
   return XG_OK;
 }
 
