@@ -323,7 +323,7 @@ def leg_batching(args) -> dict:
             "solves_per_s_batched": len(specs) / t_batch, "solves_per_s_plain": len(specs) / t_plain,
             "cell_updates_per_s_batched": cells * K / t_batch, "ms_per_batched_step": t_batch / K * 1e3,
             "identical_to_plain": bool(same), "counters": ms.counters(),
-            "tile_shapes": {k: list(v) for k, v in ms.shapes.items()},
+            "tile_shapes": {k: list(v) if v else None for k, v in ms.shapes.items()},
             "autotune": {k: {"timings_ms_per_step": v["timings"], "cached": v["cached"]} for k, v in tuning.items()},
             "plain_counters": {k: {c: v - plain_before[k][c] for c, v in op.native.counters().items()}
                                for k, op in ms.ops.items()}}
@@ -361,7 +361,7 @@ def leg_serving(args) -> dict:
                         f"{args.serve_tick} steps per tick",
             "cell_steps": work, "batched": summ, "plain_fifo": summ_plain,
             "speedup_solves_per_s": summ["solves_per_s"] / summ_plain["solves_per_s"],
-            "identical_to_plain": bool(same), "tile_shapes": {k: list(v) for k, v in srv.ms.shapes.items()},
+            "identical_to_plain": bool(same), "tile_shapes": {k: list(v) if v else None for k, v in srv.ms.shapes.items()},
             "autotune": {k: {"timings_ms_per_step": v["timings"], "cached": v["cached"]} for k, v in tuning.items()},
             "below_saturation": dict(summ_low, rate_per_s=args.serve_low_rate,
                                      workload=f"{len(low)} requests, seed 3, Poisson {args.serve_low_rate}/s")}

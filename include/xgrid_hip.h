@@ -85,6 +85,11 @@ typedef struct xg_loop_desc {
   void* fn_check;                            /* the same, proving every written core cell exact */
   void* fn_b;                                /* batched variants */
   void* fn_check_b;
+  int32_t alt;                               /* plain calls on a grid of at most alt_max_cells cells run
+                                                loop plan `alt` (another tile shape of the same loop)
+                                                instead; -1: none */
+  int32_t alt_pad;
+  int64_t alt_max_cells;
 } xg_loop_desc;
 
 /* Function table handed to generated host stubs (the stub runs the kernel's

@@ -374,7 +374,7 @@ def test_blocked_loop_declined_when_halo_too_small(double):
 
 
 @pytest.mark.parametrize("shape", [(40, 64, 512, 6, "reg"), (24, 64, 512, 4, "reg"), (48, 64, 768, 10, "reg"),
-                                   (32, 64, 512, 8, "lds"), (24, 64, 256, 6, "lds")])
+                                   (32, 64, 512, 8, "lds"), (24, 64, 256, 6, "lds"), (72, 64, 1024, 10, "lds")])
 def test_blocked_loop_shapes_and_layouts_match_oracle(double, shape):
     """Every tile shape and both layouts (region in LDS / columns in registers)
     of the blocked loop give the oracle's bits, on grids whose edges cut tiles."""
@@ -392,6 +392,28 @@ def test_blocked_loop_shapes_and_layouts_match_oracle(double, shape):
             assert_bitexact(*pairs[c], levels=[0])
         info = k.native.loop_info(0)
         assert info["state"] == 1 and info["iterations"] == 50 * steps
+
+
+def test_fp32_default_tiles_by_grid_size(single):
+    """With FP32 cells the default build holds two tiles for the blocked loop: plain
+    calls on grids of at most 724^2 cells run the 768-work-item one (loop plan 1),
+    larger grids the 1024-work-item one (plan 0).  Both give the oracle's bits."""
+    from xgrid_amd import workloads as W
+    from xgrid_amd.emit import Emitter
+    k = W.make_cavity()
+    for n, plan, steps in ((97, 1, 2), (730, 0, 1)):
+        assert (n * n <= Emitter.ALT_MAX_CELLS) == (plan == 1)
+        cfg = W.cavity_config(n, dt=2e-6)
+        masks = W.cavity_masks(n)
+        pairs = {c: grid_pair(xgrid, (n, n), float, masks[c]) for c in "bpuv"}
+        before = k.native.loop_info(plan)["iterations"] if k.native is not None else 0
+        run_pair(k, [pairs[c][0] for c in "bpuv"] + [cfg], [pairs[c][1] for c in "bpuv"] + [cfg], steps=steps)
+        for c in "bpuv":
+            assert_bitexact(*pairs[c], levels=[0])
+        L = k.native.lowered.loops[0]
+        assert L.threads == 1024 and L.alt is not None and L.alt.threads == 768
+        info = k.native.loop_info(plan)
+        assert info["state"] == 1 and info["iterations"] - before == 50 * steps
 
 
 def test_register_layout_declines_when_halo_too_small(double):

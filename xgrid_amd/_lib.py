@@ -48,7 +48,7 @@ class LoopDesc(Structure):
     _fields_ = [("nstmts", c_int32), ("sids", c_int32 * XG_LOOP_MAX), ("grid_arg", c_int32),
                 ("core0", c_int32), ("core1", c_int32), ("halo", c_int32), ("tmax", c_int32),
                 ("threads", c_int32), ("fn", c_void_p), ("fn_check", c_void_p), ("fn_b", c_void_p),
-                ("fn_check_b", c_void_p)]
+                ("fn_check_b", c_void_p), ("alt", c_int32), ("alt_pad", c_int32), ("alt_max_cells", c_int64)]
 
 
 class Api(Structure):

@@ -24,6 +24,7 @@ from .emit import Emitter
 
 CANDIDATES = [
     (48, 64, 768, 10, "lds"),
+    (72, 64, 1024, 10, "lds"),
     (40, 64, 512, 10, "lds"),
     (32, 64, 512, 8, "lds"),
     (24, 64, 256, 6, "lds"),
@@ -40,7 +41,7 @@ def has_blocked_loop(op) -> bool:
 
 def _key(op, precision: str, sizes: list, candidates: list) -> str:
     from .config import get_config
-    low, _ = lower_kernel(op, Emitter.DEFAULT_BLOCK_SHAPE)
+    low, _ = lower_kernel(op, Emitter.DEFAULT_BLOCK_SHAPE)   # any fixed shape: the key names the kernel
     lib = _lib.load()
     import ctypes
     buf = ctypes.create_string_buffer(256)
@@ -123,7 +124,7 @@ def tune_batch(op, make_instance, instances: list, precision: str, candidates: l
         b.close()
     valid = {k: v for k, v in timings.items() if v is not None}
     if not valid:
-        return {"shape": Emitter.DEFAULT_BLOCK_SHAPE, "timings": timings, "cached": False}
+        return {"shape": None, "timings": timings, "cached": False}
     best = min(valid, key=valid.get)
     parts = best.split(",")
     shape = tuple(int(x) for x in parts[:4]) + (parts[4],)

@@ -59,7 +59,7 @@ class HipKernel:
         self.op = op
         self.lib = _lib.ensure_device(cfg.device)
         low, info = lower_kernel(op, block_shape)
-        self.block_shape = Emitter.block_shape() if block_shape is None else tuple(block_shape)
+        self.block_shape = tuple(block_shape) if block_shape is not None else low.block_shape
         self.lowered = low
         self.info = info
         self.depth = info.depth
@@ -122,8 +122,12 @@ class HipKernel:
             groups[i].count = g.count
             groups[i].fn = fn(g.kernel)
             groups[i].fn_b = fn(g.kernel + "_b")
-        loops = (_lib.LoopDesc * max(1, len(low.loops)))()
-        for i, L in enumerate(low.loops):
+        # loop plans: one per blockable loop (ids as in the stub), then the small-grid
+        # tiles of those that have one
+        alts = [L.alt for L in low.loops if L.alt is not None]
+        loops = (_lib.LoopDesc * max(1, len(low.loops) + len(alts)))()
+        nalt = 0
+        for i, L in enumerate(list(low.loops) + alts):
             ld = loops[i]
             ld.nstmts = len(L.stmts)
             for j, m in enumerate(L.stmts):
@@ -135,8 +139,14 @@ class HipKernel:
             ld.threads = L.threads
             ld.fn = fn(L.kernel)
             ld.fn_check = fn(L.kernel + "_chk")
-            ld.fn_b = fn(L.kernel + "_b")
-            ld.fn_check_b = fn(L.kernel + "_chkb")
+            ld.fn_b = fn(L.kernel + "_b") if L.batched else None
+            ld.fn_check_b = fn(L.kernel + "_chkb") if L.batched else None
+            ld.alt = -1
+            if i < len(low.loops) and L.alt is not None:
+                ld.alt = len(low.loops) + nalt
+                ld.alt_max_cells = Emitter.ALT_MAX_CELLS
+                nalt += 1
+        self._nloops = len(low.loops) + len(alts)
         self._descs, self._groups, self._loops = descs, groups, loops
         self.kernel = self.new_plan()
         self.set_options()
@@ -160,7 +170,7 @@ class HipKernel:
         low = self.lowered
         k = c_void_p()
         _lib.check(self.lib.xg_kernel_create(len(low.stmts), self._descs, len(low.groups), self._groups,
-                                             len(low.loops), self._loops, len(low.grid_args), ctypes.byref(k)),
+                                             self._nloops, self._loops, len(low.grid_args), ctypes.byref(k)),
                    "xg_kernel_create")
         return k.value
 

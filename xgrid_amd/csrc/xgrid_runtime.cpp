@@ -629,6 +629,10 @@ int api_loop(xg_kernel* k, int32_t lid, int64_t trip, const void* const* params,
     set_status(k, fail(XG_ERR_ARG, "bad loop description"));
     return 1;
   }
+  if (k->mode != kRecord) {  // plain calls on small grids take the loop's other tile shape
+    const xg_loop_desc& d = k->loops[(size_t)lid].d;
+    if (d.alt >= 0 && k->grids[(size_t)d.grid_arg]->n <= d.alt_max_cells) lid = d.alt;
+  }
   LoopPlan& L = k->loops[(size_t)lid];
   if (k->mode == kRecord) {
     Rec r{1, lid, trip, k->rec_arena.size()};
@@ -1046,6 +1050,11 @@ int xg_kernel_create(int32_t nstmts, const xg_stmt_desc* stmts, int32_t ngroups,
     for (int j = 0; ok && j < d.nstmts; ++j)
       ok = d.sids[j] >= 0 && d.sids[j] < nstmts && k->stmts[(size_t)d.sids[j]].d.store_arg == d.grid_arg &&
            k->stmts[(size_t)d.sids[j]].d.dims == 2;
+    if (ok && d.alt >= 0) {  // the alternative runs the same statements and has no alternative itself
+      ok = d.alt < nloops && d.alt != i && loops[d.alt].alt < 0 && loops[d.alt].nstmts == d.nstmts &&
+           loops[d.alt].grid_arg == d.grid_arg;
+      for (int j = 0; ok && j < d.nstmts; ++j) ok = loops[d.alt].sids[j] == d.sids[j];
+    }
     if (!ok) {
       delete k;
       return fail(XG_ERR_ARG, "invalid loop description " + std::to_string(i));

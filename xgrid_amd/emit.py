@@ -21,6 +21,7 @@ otherwise.
 """
 from __future__ import annotations
 
+import os
 from dataclasses import dataclass, field
 
 from . import ir
@@ -104,6 +105,8 @@ class LoopMeta:
     threads: int = 512
     reach: int = 1                # largest |offset| of the body's reads of the loop grid
     variant: str = "lds"          # "lds": region in LDS; "reg": one column per lane in registers
+    alt: object = None            # LoopMeta of the same loop with the tile used on small grids
+    batched: bool = True          # emit the batched variants (the small-grid tile has none)
 
 
 @dataclass
@@ -116,6 +119,7 @@ class Lowered:
     scalar_args: list             # [(arg index, name, type)]
     graphs_ok: bool
     loops: list = field(default_factory=list)
+    block_shape: tuple | None = None   # shape of the blocked loops (None: the kernel has none)
 
 
 class _ExprC:
@@ -200,7 +204,8 @@ class Emitter:
     def __init__(self, info: KernelInfo, overstep: str = "none", comment: bool = False,
                  entry: str = "xg_entry", macro: list | None = None, block_shape: tuple | None = None):
         self.info = info
-        self.shape = tuple(block_shape) if block_shape is not None else self.block_shape()
+        self.shape = tuple(block_shape) if block_shape is not None else None   # None: by element size
+        self.used_shape = None         # the shape the blockable loops were built with
         self.macro = list(macro or [])
         self.func = info.func
         self.overstep = overstep
@@ -628,17 +633,33 @@ class Emitter:
           f"{{ {L.kernel}_body<false>({call}); }}")
         o(f'extern "C" __global__ void {self._tb_bounds(L, True)} {L.kernel}_chk({", ".join(kargs)}) '
           f"{{ {L.kernel}_body<true>({call}); }}")
-        self._tb_batched(L, out)
+        if L.batched:
+            self._tb_batched(L, out)
 
     @staticmethod
     def _tb_bounds(L: LoopMeta, check: bool = False) -> str:
-        """Launch bounds of a blocked kernel: two workgroups per CU (its LDS or
-        registers allow no more), so each SIMD must hold threads / 128 waves; the
-        attribute keeps the compiler's register budget at that occupancy.  The
-        checking variant runs once per plan and needs more LDS: no target."""
+        """Launch bounds of a blocked kernel, with the register budget set for the
+        occupancy its LDS allows (see _tb_waves_per_simd).  The checking variant runs
+        once per plan and needs more LDS: no target."""
         if check:
             return f"__launch_bounds__({L.threads})"
-        return f"__launch_bounds__({L.threads}) __attribute__((amdgpu_waves_per_eu({max(1, L.threads // 128)})))"
+        return (f"__launch_bounds__({L.threads}) "
+                f"__attribute__((amdgpu_waves_per_eu({Emitter._tb_waves_per_simd(L)})))")
+
+    @staticmethod
+    def _tb_waves_per_simd(L: LoopMeta) -> int:
+        """Waves per SIMD the register budget is set for: two workgroups per CU, or
+        one when the LDS region (160 KiB per CU) leaves room for only one.  Asking
+        for more than two made the compiler spill on the 512-work-item shapes."""
+        waves = L.threads // 64
+        wg = 2
+        if L.variant == "lds":
+            elem = 8 if device_type_name(L.stmts[0].stmt.gtype.element) == "double" else 4
+            nr = (L.core[0] + 2 * L.halo) * (L.core[1] + 2 * L.halo)
+            sl = nr // 4 if nr // 4 > 256 else (nr if nr < 256 else 256)
+            lds = 2 * nr * elem + nr + 2 * sl + 64       # both region copies, membership bytes, sparse list
+            wg = max(1, min(wg, 163840 // lds))
+        return max(1, min(8, -(-wg * waves // 4)))
 
     def _tb_batched(self, L: LoopMeta, out: list):
         """Batched variants of a blocked loop's kernel: the tiles of many instances.
@@ -828,7 +849,8 @@ class Emitter:
           f"{{ {L.kernel}_body<false>({call}); }}")
         o(f'extern "C" __global__ void {self._tb_bounds(L, True)} {L.kernel}_chk({", ".join(kargs)}) '
           f"{{ {L.kernel}_body<true>({call}); }}")
-        self._tb_batched(L, out)
+        if L.batched:
+            self._tb_batched(L, out)
 
     def device_source(self, stmt_meta: list, groups: list, loops: list = ()) -> str:
         out = ["// generated by xgrid_amd for the MI355X backend", '#include "xgrid_device.h"',
@@ -846,7 +868,9 @@ class Emitter:
         for m in stmt_meta:
             self._device_stmt(m.stmt, m.refs, m.params, out, loop_of.get(m.stmt.sid))
         for L in loops:
-            (self._tb_kernel_reg if L.variant == "reg" else self._tb_kernel_lds)(L, out)
+            for V in (L, L.alt):
+                if V is not None:
+                    (self._tb_kernel_reg if V.variant == "reg" else self._tb_kernel_lds)(V, out)
         # phase-2 kernels per (element type, dimension)
         done = set()
         for m in stmt_meta:
@@ -1107,24 +1131,26 @@ class Emitter:
         return Lowered(self.device_source(stmt_meta, groups, loops), self.stub_source(stmt_meta, loops), stmt_meta,
                        groups, list(self.func.grid_args),
                        [(i, n, t) for i, (n, t) in enumerate(self.func.args) if not isinstance(t, GridType)],
-                       graphs_ok, loops)
+                       graphs_ok, loops, self.used_shape)
 
     PREFETCH_MAX = 4   # most other-grid loads of a blocked statement kept in registers per cell
 
     DEFAULT_BLOCK_SHAPE = (48, 64, 768, 10, "lds")
+    # 4-byte cells: the 96 x 64 region of 1024 work-items leaves LDS for two workgroups
+    # per CU (32 waves); 8-byte cells fit two only with the 72 x 64 region of 768
+    DEFAULT_BLOCK_SHAPE_4B = (72, 64, 1024, 10, "lds")
 
     @classmethod
-    def block_shape(cls, spec: str | None = None) -> tuple:
+    def block_shape(cls, spec: str | None = None, elem: int = 8) -> tuple:
         """(core rows, region columns, work-items, most iterations per launch, variant)
         of blocked loops.  The region (core + halo) is a power of two wide so a region
         index splits into row and column with shifts; the core is the region width less
         the halo.  Variant "lds" keeps the region in LDS; "reg" keeps one region column
         per lane in registers (64 columns).  XGRID_TB_SHAPE="rows,cols,threads,tmax[,variant]"
-        overrides the default."""
-        import os
+        overrides the default, which depends on the loop grid's element size `elem`."""
         spec = spec if spec is not None else os.environ.get("XGRID_TB_SHAPE")
         if not spec:
-            return cls.DEFAULT_BLOCK_SHAPE
+            return cls.DEFAULT_BLOCK_SHAPE_4B if elem == 4 else cls.DEFAULT_BLOCK_SHAPE
         parts = spec.split(",")
         r, c, t, m = (int(x) for x in parts[:4])
         v = parts[4].strip() if len(parts) > 4 else "lds"
@@ -1157,25 +1183,43 @@ class Emitter:
                 for c in ir.walk(m.stmt.node.value):
                     if isinstance(c, ir.Cell) and c.grid.name == g and c.level == 0:
                         reach = max(reach, max(abs(o) for o in c.offsets))
-            rows, region_cols, threads, tm, variant = self.shape
-            tmax = max(1, tm // max(1, reach)) if reach else tm
-            # information moves `reach` cells per iteration through the bulk; a boundary
-            # statement reading a neighbour computed in the same iteration adds a hop,
-            # so leave room for two such hops (a validation launch proves the choice)
-            halo = tmax * reach + 2 * reach
-            cols = region_cols - 2 * halo
-            if halo > 16 or cols < 16 or threads % 64 or threads > 1024:
+            elem = 8 if device_type_name(ms[0].stmt.gtype.element) == "double" else 4
+            shape = self.shape if self.shape is not None else self.block_shape(elem=elem)
+            L = self._loop_meta(len(loops), n, ms, g, reach, shape, f"xg_tb{len(loops)}")
+            if L is None:
                 continue
-            if variant == "reg":
-                waves = threads // 64
-                if region_cols != 64 or reach < 1:
-                    continue
-                rows += (-(rows + 2 * halo)) % waves      # whole region rows per wave
-                if (rows + 2 * halo) // waves > 16:
-                    continue
-            loops.append(LoopMeta(len(loops), n, ms, g, (rows, cols), halo, tmax, f"xg_tb{len(loops)}", threads,
-                                  reach, variant))
+            if self.shape is None and elem == 4 and not os.environ.get("XGRID_TB_SHAPE"):
+                # plain calls on smaller grids run the 768-work-item tile (its latency per
+                # launch is lower; the 1024 one wins from about 724^2 cells up)
+                L.alt = self._loop_meta(len(loops), n, ms, g, reach, self.DEFAULT_BLOCK_SHAPE, f"xg_tb{len(loops)}_s")
+                if L.alt is not None:
+                    L.alt.batched = False
+            loops.append(L)
+            self.used_shape = tuple(shape)
         return loops
+
+    ALT_MAX_CELLS = 724 * 724   # grids of at most this many cells take a loop's small-grid tile
+
+    @staticmethod
+    def _loop_meta(lid: int, node, ms: list, g: str, reach: int, shape: tuple, name: str):
+        """The blocked kernel of one loop for one tile shape, or None if the shape cannot hold it."""
+        rows, region_cols, threads, tm, variant = shape
+        tmax = max(1, tm // max(1, reach)) if reach else tm
+        # information moves `reach` cells per iteration through the bulk; a boundary
+        # statement reading a neighbour computed in the same iteration adds a hop,
+        # so leave room for two such hops (a validation launch proves the choice)
+        halo = tmax * reach + 2 * reach
+        cols = region_cols - 2 * halo
+        if halo > 16 or cols < 16 or threads % 64 or threads > 1024:
+            return None
+        if variant == "reg":
+            waves = threads // 64
+            if region_cols != 64 or reach < 1:
+                return None
+            rows += (-(rows + 2 * halo)) % waves      # whole region rows per wave
+            if (rows + 2 * halo) // waves > 16:
+                return None
+        return LoopMeta(lid, node, ms, g, (rows, cols), halo, tmax, name, threads, reach, variant)
 
     def _groups(self, stmt_meta: list) -> list:
         """Runs of consecutive stencil statements (same block, nothing between)."""
