@@ -335,7 +335,7 @@ def test_blocked_loop_matches_unblocked_and_oracle(double):
         run_pair(k, [pairs[c][0] for c in "bpuv"] + [cfg], [pairs[c][1] for c in "bpuv"] + [cfg], steps=steps)
         for c in "bpuv":
             assert_bitexact(*pairs[c], levels=[0])
-        info = k.native.loop_info(0)
+        info = k.native.loop_info(k.native.loop_plan(0, n * n))
         if blocked:
             assert info["state"] == 1 and info["iterations"] == 50 * steps
         else:
@@ -369,7 +369,7 @@ def test_blocked_loop_declined_when_halo_too_small(double):
     du, hu = grid_pair(xgrid, (n, n), float, mask, rng.random((n, n)))
     dv, hv = grid_pair(xgrid, (n, n), float, None, rng.random((n, n)) * 1e-3)
     run_pair(creep, (du, dv), (hu, hv), steps=2)
-    assert creep.native.loop_info(0)["state"] == -1
+    assert creep.native.loop_info(creep.native.loop_plan(0, n * n))["state"] == -1
     assert_bitexact(du, hu)
 
 
@@ -395,14 +395,12 @@ def test_blocked_loop_shapes_and_layouts_match_oracle(double, shape):
 
 
 def test_fp32_default_tiles_by_grid_size(single):
-    """With FP32 cells the default build holds two tiles for the blocked loop: plain
-    calls on grids of at most 724^2 cells run the 768-work-item one (loop plan 1),
-    larger grids the 1024-work-item one (plan 0).  Both give the oracle's bits."""
+    """The default build holds two tiles for the blocked loop: plain calls on grids
+    of at most 800^2 FP32 cells run the small-grid tile (40 x 40 core, loop plan 1),
+    larger grids the 72 x 40 one (plan 0).  Both give the oracle's bits."""
     from xgrid_amd import workloads as W
-    from xgrid_amd.emit import Emitter
     k = W.make_cavity()
-    for n, plan, steps in ((97, 1, 2), (730, 0, 1)):
-        assert (n * n <= Emitter.ALT_MAX_CELLS) == (plan == 1)
+    for n, plan, steps in ((97, 1, 2), (810, 0, 1)):
         cfg = W.cavity_config(n, dt=2e-6)
         masks = W.cavity_masks(n)
         pairs = {c: grid_pair(xgrid, (n, n), float, masks[c]) for c in "bpuv"}
@@ -411,7 +409,8 @@ def test_fp32_default_tiles_by_grid_size(single):
         for c in "bpuv":
             assert_bitexact(*pairs[c], levels=[0])
         L = k.native.lowered.loops[0]
-        assert L.threads == 1024 and L.alt is not None and L.alt.threads == 768
+        assert L.core == (72, 40) and L.alt is not None and L.alt.core == (40, 40)
+        assert k.native.loop_plan(0, n * n) == plan
         info = k.native.loop_info(plan)
         assert info["state"] == 1 and info["iterations"] - before == 50 * steps
 

@@ -27,7 +27,7 @@ CANDIDATES = [
     (72, 64, 1024, 10, "lds"),
     (40, 64, 512, 10, "lds"),
     (32, 64, 512, 8, "lds"),
-    (24, 64, 256, 6, "lds"),
+    (40, 64, 1024, 10, "lds"),
     (40, 64, 512, 6, "reg"),
     (40, 64, 512, 5, "reg"),
     (24, 64, 512, 4, "reg"),

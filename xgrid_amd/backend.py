@@ -144,7 +144,7 @@ class HipKernel:
             ld.alt = -1
             if i < len(low.loops) and L.alt is not None:
                 ld.alt = len(low.loops) + nalt
-                ld.alt_max_cells = Emitter.ALT_MAX_CELLS
+                ld.alt_max_cells = L.alt_max_cells
                 nalt += 1
         self._nloops = len(low.loops) + len(alts)
         self._descs, self._groups, self._loops = descs, groups, loops
@@ -187,6 +187,15 @@ class HipKernel:
         f = cfg.fuse if fuse is None else fuse
         b = cfg.blocked if blocked is None else blocked
         _lib.check(self.lib.xg_kernel_set_options(self.kernel if plan is None else plan, int(g), int(f), int(b)))
+
+    def loop_plan(self, lid: int, cells: int) -> int:
+        """The loop plan plain calls use for loop `lid` on a grid of `cells` cells
+        (the loop's own, or its small-grid tile's)."""
+        low = self.lowered
+        L = low.loops[lid]
+        if L.alt is None or cells > L.alt_max_cells:
+            return lid
+        return len(low.loops) + sum(1 for M in low.loops[:lid] if M.alt is not None)
 
     def loop_info(self, lid: int = 0) -> dict:
         n, it, st = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int32()

@@ -106,6 +106,7 @@ class LoopMeta:
     reach: int = 1                # largest |offset| of the body's reads of the loop grid
     variant: str = "lds"          # "lds": region in LDS; "reg": one column per lane in registers
     alt: object = None            # LoopMeta of the same loop with the tile used on small grids
+    alt_max_cells: int = 0        # ... on grids of at most this many cells
     batched: bool = True          # emit the batched variants (the small-grid tile has none)
 
 
@@ -1139,6 +1140,12 @@ class Emitter:
     # 4-byte cells: the 96 x 64 region of 1024 work-items leaves LDS for two workgroups
     # per CU (32 waves); 8-byte cells fit two only with the 72 x 64 region of 768
     DEFAULT_BLOCK_SHAPE_4B = (72, 64, 1024, 10, "lds")
+    # Plain calls on smaller grids take a second tile: 1024 work-items on a 64 x 64
+    # region (two workgroups per CU in either precision), whose launches are shorter.
+    # Measured per cavity step: it wins up to 724^2 (FP32) and 1024^2 (FP64) cells and
+    # loses from 1024^2 (FP32) and 1448^2 (FP64) up.
+    SMALL_GRID_SHAPE = (40, 64, 1024, 10, "lds")
+    SMALL_GRID_MAX_CELLS = {4: 800 * 800, 8: 1300 * 1300}
 
     @classmethod
     def block_shape(cls, spec: str | None = None, elem: int = 8) -> tuple:
@@ -1188,17 +1195,14 @@ class Emitter:
             L = self._loop_meta(len(loops), n, ms, g, reach, shape, f"xg_tb{len(loops)}")
             if L is None:
                 continue
-            if self.shape is None and elem == 4 and not os.environ.get("XGRID_TB_SHAPE"):
-                # plain calls on smaller grids run the 768-work-item tile (its latency per
-                # launch is lower; the 1024 one wins from about 724^2 cells up)
-                L.alt = self._loop_meta(len(loops), n, ms, g, reach, self.DEFAULT_BLOCK_SHAPE, f"xg_tb{len(loops)}_s")
+            if self.shape is None and not os.environ.get("XGRID_TB_SHAPE"):
+                L.alt = self._loop_meta(len(loops), n, ms, g, reach, self.SMALL_GRID_SHAPE, f"xg_tb{len(loops)}_s")
                 if L.alt is not None:
                     L.alt.batched = False
+                    L.alt_max_cells = self.SMALL_GRID_MAX_CELLS[elem]
             loops.append(L)
             self.used_shape = tuple(shape)
         return loops
-
-    ALT_MAX_CELLS = 724 * 724   # grids of at most this many cells take a loop's small-grid tile
 
     @staticmethod
     def _loop_meta(lid: int, node, ms: list, g: str, reach: int, shape: tuple, name: str):
