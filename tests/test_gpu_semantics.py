@@ -374,7 +374,8 @@ def test_blocked_loop_declined_when_halo_too_small(double):
 
 
 @pytest.mark.parametrize("shape", [(40, 64, 512, 6, "reg"), (24, 64, 512, 4, "reg"), (48, 64, 768, 10, "reg"),
-                                   (32, 64, 512, 8, "lds"), (24, 64, 256, 6, "lds"), (72, 64, 1024, 10, "lds")])
+                                   (32, 64, 512, 8, "lds"), (24, 64, 256, 6, "lds"), (72, 64, 1024, 10, "lds"),
+                                   (48, 64, 768, 10, "lds")])
 def test_blocked_loop_shapes_and_layouts_match_oracle(double, shape):
     """Every tile shape and both layouts (region in LDS / columns in registers)
     of the blocked loop give the oracle's bits, on grids whose edges cut tiles."""

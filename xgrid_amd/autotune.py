@@ -23,6 +23,7 @@ from .backend import HipKernel, lower_kernel
 from .emit import Emitter
 
 CANDIDATES = [
+    (48, 64, 1024, 10, "lds"),
     (48, 64, 768, 10, "lds"),
     (72, 64, 1024, 10, "lds"),
     (40, 64, 512, 10, "lds"),

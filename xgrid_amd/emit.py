@@ -1136,9 +1136,10 @@ class Emitter:
 
     PREFETCH_MAX = 4   # most other-grid loads of a blocked statement kept in registers per cell
 
-    DEFAULT_BLOCK_SHAPE = (48, 64, 768, 10, "lds")
-    # 4-byte cells: the 96 x 64 region of 1024 work-items leaves LDS for two workgroups
-    # per CU (32 waves); 8-byte cells fit two only with the 72 x 64 region of 768
+    # 1024 work-items; the region (core + halo 12 on each side) takes as many rows as
+    # leave LDS for two workgroups per CU (32 waves): 72 x 64 with 8-byte cells (80.6
+    # KB each), 96 x 64 with 4-byte cells
+    DEFAULT_BLOCK_SHAPE = (48, 64, 1024, 10, "lds")
     DEFAULT_BLOCK_SHAPE_4B = (72, 64, 1024, 10, "lds")
     # Plain calls on smaller grids take a second tile: 1024 work-items on a 64 x 64
     # region (two workgroups per CU in either precision), whose launches are shorter.
