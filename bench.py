@@ -48,6 +48,8 @@ def main():
                     help="secondary workloads after the headline one: all, none, or a comma list")
     ap.add_argument("--batch-steps", type=int, default=100, help="steps of the batching leg")
     ap.add_argument("--heat-steps", type=int, default=1000, help="calls of the 3-D heat leg")
+    ap.add_argument("--batch-no-plain", action="store_true",
+                    help="batching leg: time the batched steps only (no plain comparison), e.g. under a profiler")
     ap.add_argument("--batch-max-blocks", type=int, default=0,
                     help="persistent grid cap of batched launches (0: one workgroup per tile)")
     ap.add_argument("--serve-requests", type=int, default=48, help="requests of the serving trace (400 = full)")
@@ -137,7 +139,7 @@ def main():
         kernel_name = (f"{L.kernel} (pressure loop blocked: {trip} iterations of Jacobi sweep + 4 boundary "
                        f"statements in {launches_per_trip} launches of <= {L.tmax}, {L.core[0]}x{L.core[1]} "
                        f"tiles, halo {L.halo})")
-        kernel_id = f"{L.kernel}:{L.core[0]}x{L.core[1]}:h{L.halo}:t{L.tmax}:n{n}"
+        kernel_id = f"{L.kernel}:{L.core[0]}x{L.core[1]}:h{L.halo}:t{L.tmax}:w{L.threads}:n{n}"
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
 
     name = ctypes_name(lib)
@@ -302,6 +304,13 @@ def leg_batching(args) -> dict:
         ms.step(K)
         _lib.check(lib.xg_synchronize())
         t_batch = time.perf_counter() - t0
+        key = "f64" if prec == "double" else "f32"
+        if args.batch_no_plain:
+            out[key] = {"batched_s": t_batch, "ms_per_batched_step": t_batch / K * 1e3,
+                        "cell_updates_per_s_batched": cells * K / t_batch, "counters": ms.counters(),
+                        "tile_shapes": {k: list(v) if v else None for k, v in ms.shapes.items()}}
+            ms.close()
+            continue
         plain_before = {k: op.native.counters() for k, op in ms.ops.items()}
         plain = []
         for s in specs:
@@ -318,7 +327,7 @@ def leg_batching(args) -> dict:
         t_plain = time.perf_counter() - t0
         same = all(np.array_equal(a.now, b.now) for h, (grids, _) in zip(handles, plain)
                    for a, b in zip(ms.grids(h), grids))
-        out["f64" if prec == "double" else "f32"] = {
+        out[key] = {
             "batched_s": t_batch, "plain_s": t_plain, "speedup": t_plain / t_batch,
             "solves_per_s_batched": len(specs) / t_batch, "solves_per_s_plain": len(specs) / t_plain,
             "cell_updates_per_s_batched": cells * K / t_batch, "ms_per_batched_step": t_batch / K * 1e3,

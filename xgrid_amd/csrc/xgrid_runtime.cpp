@@ -402,8 +402,14 @@ void fill_tb_hdr(xg_kernel* k, LoopPlan& L, xg_tb_hdr& TH, int iters, int32_t* f
   }
 }
 
+// An executable graph may still be running on the stream (replays are
+// asynchronous): wait for the stream before destroying it, so its kernel
+// arguments are not freed under a launch in flight.
 void destroy_entry(GraphEntry& e) {
-  if (e.exec) (void)hipGraphExecDestroy(e.exec);
+  if (e.exec) {
+    if (g_dev.ready) (void)hipStreamSynchronize(g_dev.stream);
+    (void)hipGraphExecDestroy(e.exec);
+  }
   e.exec = nullptr;
 }
 
