@@ -436,9 +436,21 @@ def cpu_baseline(n: int, dt: float, seconds: float) -> dict:
         el = time.perf_counter() - t0
         if el >= seconds or steps >= 50:
             break
+    # the same path on one thread, a shorter sample
+    ck.threads(1)
+    steps1 = 0
+    t0 = time.perf_counter()
+    while True:
+        ck(*grids, cfg)
+        steps1 += 1
+        el1 = time.perf_counter() - t0
+        if el1 >= seconds / 3 or steps1 >= 10:
+            break
     return {"value": n * n * steps / el, "unit": "cell-updates/s", "cores": threads, "kind": "port",
             "sample": f"{steps} steps of the {n}x{n} cavity ({el:.1f} s), gcc -O3 -fopenmp, "
-                      f"OMP_NUM_THREADS={threads}"}
+                      f"OMP_NUM_THREADS={threads}",
+            "single_thread": {"value": n * n * steps1 / el1, "cores": 1,
+                              "sample": f"{steps1} steps of the {n}x{n} cavity ({el1:.1f} s), one thread"}}
 
 
 if __name__ == "__main__":
