@@ -5,10 +5,12 @@ boundary classes each, u, v, walls, lid) on the MI355X backend, fields resident
 in HBM.  Prints one JSON line:
 
 * value: cell-updates/s = 2049^2 x steps / wall time of the timed steps;
-* roofline: the dominant kernel (the Jacobi pressure sweep) - algorithmic bytes
-  per launch (class-0 cells x 8 B x 3 arrays: p read, b read, p written) over
-  its average launch duration, measured with HIP events on the backend stream
-  in an instrumented pass of the same steps, against 8 TB/s;
+* roofline: the dominant kernel, the temporally blocked pressure loop (the
+  Jacobi sweep and its boundary statements, up to 10 iterations per launch) -
+  algorithmic bytes per launch (per iteration: class cells x 8 B x (arrays read
+  + 1)) over its average launch duration, measured with HIP events on the
+  backend stream over back-to-back launches, against 8 TB/s; `traffic` is the
+  HBM bytes per launch from the committed rocprofv3 counter summary;
 * cpu_baseline: the oracle's restatement of the reference CPU path (gcc
   -O3 -fopenmp) on a bounded sample of the same workload on the host cores.
 
@@ -354,9 +356,15 @@ def leg_serving(args) -> dict:
     trace = W.serving_specs(args.serve_requests, seed=2, rate=200.0)
     srv = Server("float", hbm_budget=budget, steps_per_tick=args.serve_tick)
     tuning = srv.ms.autotune(trace, limit=args.tune_limit) if args.tune_limit else {}
-    got, summ = srv.run(trace)
+    def progress(label):
+        def on_done(_, n):
+            if n % 25 == 0 or n == len(trace):
+                print(f"# serving {label}: {n} of {len(trace)} requests done", file=sys.stderr, flush=True)
+        return on_done
+
+    got, summ = srv.run(trace, on_done=progress("batched"))
     summ.update(peak_resident_requests=srv.peak_active, peak_resident_bytes=srv.peak_used, ticks=srv.ticks)
-    ref, summ_plain = serve_plain(trace, "float", solver=srv.ms)
+    ref, summ_plain = serve_plain(trace, "float", solver=srv.ms, on_done=progress("plain FIFO"))
     same = all(np.array_equal(got[r].fields[c], ref[r].fields[c]) for r in got for c in "bpuv")
     del got, ref
     low = W.serving_specs(args.serve_low_requests, seed=3, rate=args.serve_low_rate)

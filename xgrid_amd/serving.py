@@ -115,9 +115,10 @@ class Server:
         self.peak_active = 0
         self.ticks = 0
 
-    def run(self, trace: list, time_scale: float = 1.0) -> tuple:
+    def run(self, trace: list, time_scale: float = 1.0, on_done=None) -> tuple:
         """Serve `trace` (arrival times in seconds, scaled by `time_scale`) in real time;
-        returns (results by request id, summary)."""
+        returns (results by request id, summary).  `on_done(result, n_done)` is called
+        as each request completes."""
         pending = deque(sorted(trace, key=lambda s: s.arrival))
         queue: deque = deque()
         active = {}                      # handle -> [spec, steps done, admitted time]
@@ -168,6 +169,8 @@ class Server:
                 del grids
                 used -= footprint(s, self.elem)
                 results[s.rid] = Result(s.rid, s.arrival * time_scale, adm, now(), fields)
+                if on_done is not None:
+                    on_done(results[s.rid], len(results))
         self.timer.sync()
         wall = now()
         busy = self.timer.busy_seconds()
@@ -178,8 +181,9 @@ class Server:
 
 
 def serve_plain(trace: list, precision: str = "float", time_scale: float = 1.0, keep_fields: bool = True,
-                solver: MultiSolver | None = None) -> tuple:
-    """The same trace served one request at a time with plain kernel calls (FIFO)."""
+                solver: MultiSolver | None = None, on_done=None) -> tuple:
+    """The same trace served one request at a time with plain kernel calls (FIFO);
+    `on_done(result, n_done)` is called as each request completes."""
     ms = solver or MultiSolver(precision)
     lib = _lib.load()
     results = {}
@@ -202,6 +206,8 @@ def serve_plain(trace: list, precision: str = "float", time_scale: float = 1.0, 
             fields = {c: g.now.copy() for c, g in zip(names, grids)}
         del grids
         results[s.rid] = Result(s.rid, s.arrival * time_scale, adm, now(), fields)
+        if on_done is not None:
+            on_done(results[s.rid], len(results))
     wall = now()
     if solver is None:
         ms.close()
