@@ -59,6 +59,8 @@ def main():
                     help="instances the autotuner times per kernel in the batching/serving legs (0: no tuning)")
     ap.add_argument("--serve-budget-gib", type=float, default=16.0)
     ap.add_argument("--serve-tick", type=int, default=8, help="kernel calls per serving tick")
+    ap.add_argument("--serve-max-mcells", type=float, default=64.0,
+                    help="serving: cap on the admitted requests' cells, in millions (0: the HBM budget alone)")
     ap.add_argument("--serve-low-rate", type=float, default=2.0, help="arrival rate of the below-saturation run")
     ap.add_argument("--serve-low-requests", type=int, default=12)
     args = ap.parse_args()
@@ -354,7 +356,8 @@ def leg_serving(args) -> dict:
     from xgrid_amd.serving import Server, serve_plain
     budget = int(args.serve_budget_gib * (1 << 30))
     trace = W.serving_specs(args.serve_requests, seed=2, rate=200.0)
-    srv = Server("float", hbm_budget=budget, steps_per_tick=args.serve_tick)
+    srv = Server("float", hbm_budget=budget, steps_per_tick=args.serve_tick,
+                 max_cells=int(args.serve_max_mcells * 1e6) if args.serve_max_mcells else None)
     tuning = srv.ms.autotune(trace, limit=args.tune_limit) if args.tune_limit else {}
     def progress(label):
         def on_done(_, n):
@@ -375,7 +378,7 @@ def leg_serving(args) -> dict:
     work = sum(s.n * s.n * s.steps for s in trace)
     return {"workload": f"{len(trace)} requests of the seed-2 trace (Poisson 200/s, cavity FP32, side log-uniform "
                         f"[128, 2048], 100-2000 steps, nu in [0.001, 0.1]), HBM budget {args.serve_budget_gib} GiB, "
-                        f"{args.serve_tick} steps per tick",
+                        f"resident-cell cap {args.serve_max_mcells:g}M, {args.serve_tick} steps per tick",
             "cell_steps": work, "batched": summ, "plain_fifo": summ_plain,
             "speedup_solves_per_s": summ["solves_per_s"] / summ_plain["solves_per_s"],
             "identical_to_plain": bool(same), "tile_shapes": {k: list(v) if v else None for k, v in srv.ms.shapes.items()},

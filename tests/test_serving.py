@@ -82,6 +82,42 @@ def test_budget_bounds_the_resident_set():
     assert srv.peak_active <= 2
 
 
+def test_cell_cap_bounds_the_resident_work():
+    """With a resident-cell cap, admitted requests never hold more cells than the
+    cap (one request alone may exceed it), requests are still admitted in arrival
+    order, and every one gets exactly its steps."""
+    from xgrid_amd.serving import cells
+    trace = _trace(12)
+    cap = 2 * max(cells(s) for s in trace)
+    fake = FakeSolver()
+    srv = Server("float", hbm_budget=1 << 30, steps_per_tick=2, solver=fake, timer=FakeTimer(), keep_fields=False,
+                 max_cells=cap)
+    resident = []
+    orig_add, orig_remove = fake.add, fake.remove
+    live = {}
+
+    def add(spec):
+        h = orig_add(spec)
+        live[h] = cells(spec)
+        resident.append(sum(live.values()))
+        return h
+
+    def remove(h):
+        live.pop(h)
+        return orig_remove(h)
+
+    fake.add, fake.remove = add, remove
+    results, _ = srv.run(trace)
+    assert len(results) == 12 and max(resident) <= cap
+    assert fake.order == sorted(fake.order)
+    for s in trace:
+        assert fake.steps[s.rid] == s.steps
+    big = [W.InstanceSpec("cavity", 300, 0.05, 1e-5, steps=2, arrival=0.0, rid=0)]
+    results, _ = Server("float", hbm_budget=1 << 30, solver=FakeSolver(), timer=FakeTimer(), keep_fields=False,
+                        max_cells=10).run(big)
+    assert len(results) == 1
+
+
 def test_request_larger_than_budget_is_refused():
     trace = [W.InstanceSpec("cavity", 2048, 0.01, 1e-6, steps=5, arrival=0.0, rid=0)]
     srv = Server("float", hbm_budget=1 << 20, solver=FakeSolver(), timer=FakeTimer(), keep_fields=False)
