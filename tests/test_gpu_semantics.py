@@ -288,6 +288,39 @@ def test_graphs_and_fusion_do_not_change_results(double):
             assert np.array_equal(a, b)
 
 
+def test_graph_cache_eviction_keeps_results(double):
+    """More live grid sets than the kernel's graph cache holds (16 entries): each
+    set is called 14 times in a row (its buffer states repeat within 6 calls and a
+    state's graph is captured on its second sighting, so the last calls replay),
+    and entries of earlier sets are evicted while recent replays may still run.
+    Every set ends equal to the same calls without graphs."""
+    from xgrid_amd import workloads as W
+    n, sets, rounds, calls = 33, 20, 2, 14
+    cfg = W.cavity_config(n, dt=2e-6)
+    masks = W.cavity_masks(n)
+    res = {}
+    for graphs in (True, False):
+        xgrid.init(precision="double", graphs=graphs)
+        k = W.make_cavity()
+        gsets = []
+        for i in range(sets):
+            gs = [xgrid.Grid((n, n), float) for _ in "bpuv"]
+            for g, c in zip(gs, "bpuv"):
+                g.boundary[...] = masks[c]
+            gs[2].now[n // 2, 1 + i % (n - 2)] = 1.0      # a different start per set
+            gsets.append(gs)
+        for _ in range(rounds):
+            for gs in gsets:
+                for _ in range(calls):
+                    k(*gs, cfg)
+        res[graphs] = [[g.now.copy() for g in gs] for gs in gsets]
+        if graphs:
+            assert k.native.counters()["graph_replays"] > 0
+    for a_set, b_set in zip(res[True], res[False]):
+        for a, b in zip(a_set, b_set):
+            assert np.array_equal(a, b)
+
+
 def test_pointer_argument(double):
     @xgrid.kernel()
     def k(u: f1, s: xgrid.ptr[float]) -> None:
