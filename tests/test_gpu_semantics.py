@@ -321,6 +321,34 @@ def test_graph_cache_eviction_keeps_results(double):
             assert np.array_equal(a, b)
 
 
+def test_division_by_scalar_is_ieee_for_every_double(double):
+    """A device division must give the bits of IEEE a / c (the reference's C
+    divides with the CPU's correctly rounded division): random bit patterns (all
+    exponents, subnormals, infinities, NaNs), ordinary values and signed zeros,
+    over divisors of every size and sign, and zero."""
+    @xgrid.kernel()
+    def divk(u: f1, a: f1, c: float) -> None:
+        u[0] = a[0][0] / c
+
+    rng = np.random.default_rng(21)
+    n = 1 << 20
+    vals = rng.integers(0, 2**63, n, dtype=np.int64).view(np.float64).copy()
+    vals[: n // 2] = rng.random(n // 2) * 10.0 ** rng.integers(-30, 30, n // 2)
+    vals[::3] *= -1.0
+    vals[:8] = [0.0, -0.0, np.inf, -np.inf, np.nan, 5e-324, 1.7976931348623157e308, 2.0 ** -900]
+    a = xgrid.Grid((n,), float)
+    a.now[:] = vals
+    u = xgrid.Grid((n,), float)
+    for c in (3.0, 0.1, -7.25, 1e-5, 6.02e23, 2.0 ** -120, 2.0 ** 150, -1e300, 0.0):
+        divk(u, a, c)
+        with np.errstate(all="ignore"):
+            want = vals / c
+        got = u.now
+        nan = np.isnan(want)
+        assert np.array_equal(np.isnan(got), nan), c
+        assert np.array_equal(got[~nan].view(np.uint64), want[~nan].view(np.uint64)), c
+
+
 def test_pointer_argument(double):
     @xgrid.kernel()
     def k(u: f1, s: xgrid.ptr[float]) -> None:
