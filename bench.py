@@ -457,11 +457,29 @@ def cpu_baseline(n: int, dt: float, seconds: float) -> dict:
         el1 = time.perf_counter() - t0
         if el1 >= seconds / 3 or steps1 >= 10:
             break
+    # the 1-D diffusion (4096 cells FP64) per call on the same path: Python, argument
+    # serialisation and one OpenMP nest per statement, as the reference runs it
+    kd = W.make_diffusion_1d()
+    ckd = CpuKernel(kd, opt_level=3)
+    ckd.threads(threads)
+    u0, dmask, dx = W.diffusion_1d_setup(4096)
+    ud = HostGrid((4096,), float)
+    ud.now[...] = u0
+    ud.boundary[...] = dmask
+    ddt = 0.2 * dx * dx / 0.3
+    for _ in range(20):
+        ckd(ud, 0.3, ddt, dx)
+    t0 = time.perf_counter()
+    for _ in range(500):
+        ckd(ud, 0.3, ddt, dx)
+    d_us = (time.perf_counter() - t0) / 500 * 1e6
     return {"value": n * n * steps / el, "unit": "cell-updates/s", "cores": threads, "kind": "port",
             "sample": f"{steps} steps of the {n}x{n} cavity ({el:.1f} s), gcc -O3 -fopenmp, "
                       f"OMP_NUM_THREADS={threads}",
             "single_thread": {"value": n * n * steps1 / el1, "cores": 1,
-                              "sample": f"{steps1} steps of the {n}x{n} cavity ({el1:.1f} s), one thread"}}
+                              "sample": f"{steps1} steps of the {n}x{n} cavity ({el1:.1f} s), one thread"},
+            "diffusion_1d": {"us_per_call": d_us, "cores": threads,
+                             "sample": "500 calls of the 1-D diffusion, 4096 cells FP64"}}
 
 
 if __name__ == "__main__":
