@@ -50,11 +50,13 @@ def test_cavity_full_size_few_steps(double):
         assert_bitexact(*pairs[c], levels=[0])
 
 
+@pytest.mark.parametrize("n", [34, 130])
 @pytest.mark.parametrize("prec", ["double", "float"])
-def test_heat_3d(prec, request):
+def test_heat_3d(prec, n, request):
+    """130^3: three column tiles (the last partial) and two groups of row bands."""
     xg = request.getfixturevalue("double" if prec == "double" else "single")
     from xgrid_amd import workloads as W
-    n, steps = 34, 25
+    steps = 25 if n < 100 else 6
     dt_np = np.float64 if prec == "double" else np.float32
     u0, mask, dx = W.heat_3d_setup(n, dt_np, seed=3)
     nu = 0.1

@@ -249,6 +249,38 @@ def test_list_and_masked_classes(double):
     assert_bitexact(dv, hv)
 
 
+@pytest.mark.parametrize("shape", [(9, 11, 150), (6, 7, 64), (5, 40, 200)])
+def test_3d_classes_at_any_column(double, shape):
+    """3-D column tiles start at multiples of 64 columns.  A box class and a box
+    with holes that start and end mid-row, and the remaining interior, are each
+    updated exactly once and only inside their class; the class reading its own
+    cells at time 0 runs compute-then-store."""
+    @xgrid.kernel()
+    def k(u: f3, v: f3) -> None:
+        u[0, 0, 0] = v[0, 0, 1] + v[-1, 0, 0] * 0.5 + v[0, 1, 0] * 0.25
+        with xgrid.boundary(2):
+            u[0, 0, 0] = v[0, 0, -1] * 2.0 - v[1, 0, 0]
+        with xgrid.boundary(4):
+            u[0, 0, 0] = u[0, 0, 1][0] + v[0, -1, 1]
+
+    rng = np.random.default_rng(11)
+    n0, n1, n2 = shape
+    mask = np.full(shape, 9, np.int32)
+    mask[1:-1, 1:-1, 1:-1] = 0
+    h1 = n1 // 2
+    mask[1:-1, 1:h1, 5:n2 - 3] = 2                                     # a box from column 5
+    holes = rng.random((n0 - 2, n1 - 1 - h1, n2 - 1 - (n2 // 2 + 3))) < 0.3
+    sub = mask[1:-1, h1:-1, n2 // 2 + 3:-1]
+    sub[...] = np.where(holes, 0, 4)                                   # mid-row, with holes
+    du, hu = grid_pair(xgrid, shape, float, mask, rng.random(shape))
+    dv, hv = grid_pair(xgrid, shape, float, None, rng.random(shape))
+    run_pair(k, (du, dv), (hu, hv), steps=3)
+    assert du.class_info(2)["kind"] == "box" and du.class_info(2)["lo"][2] == 5
+    assert du.class_info(4)["kind"] == "masked"
+    assert_bitexact(du, hu)
+    assert_bitexact(dv, hv)
+
+
 def test_float_precision_mixed_literals(single):
     """precision='float': literals are C doubles, so arithmetic is mixed (F11)."""
     @xgrid.kernel()

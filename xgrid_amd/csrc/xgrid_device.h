@@ -43,7 +43,8 @@ namespace xg {
  *   1-D: 1024 consecutive cells (work-item t: t, t+256, t+512, t+768)
  *   2-D: 4 rows x 64 columns (one cell per work-item)
  *   3-D: XG_TILE3_CHUNKS x 4 planes x 4 rows x 64 columns (work-item (r, c):
- *        planes 0..3 of each chunk, chunk after chunk) */
+ *        planes 0..3 of each chunk, chunk after chunk); column tiles are
+ *        aligned to multiples of 64 columns, so the first may start below lo */
 template <int D> struct Tile;
 template <> struct Tile<1> { static constexpr int E0 = 1024, E1 = 1, E2 = 1, NC = 4, CH = 1; };
 template <> struct Tile<2> { static constexpr int E0 = 4, E1 = 64, E2 = 1, NC = 1, CH = 1; };
@@ -184,26 +185,46 @@ struct TileCells {
     typedef Tile<D> TL;
     const int tid = threadIdx.x;
     int t0 = (int)tile, t1 = 0, t2 = 0;
+    /* 3-D column tiles start at multiples of 64 columns, not at lo[2]:
+     * with a row pitch that is a multiple of the 128-byte cache line, a wave's
+     * row segment then covers whole lines instead of straddling one more, and
+     * a line is fetched once instead of in parts by two tiles (512^3 FP32:
+     * 312 against 352 us).  Columns of the first tile below lo[2] are not in
+     * the tile. */
+    const int a2 = D > 2 ? (lo[2] & ~(TL::E2 - 1)) : 0;
     if (D == 2) {
       const int n1 = (hi[1] - lo[1] + TL::E1 - 1) / TL::E1;
       t0 = (int)tile / n1;
       t1 = (int)tile - t0 * n1;
     } else if (D == 3) {
-      /* rows slowest, planes in the middle: tiles that share a halo plane run
-       * a few tiles apart, while the planes are still in the XCD's L2 */
-      const int n2 = (hi[2] - lo[2] + TL::E2 - 1) / TL::E2;
-      const int n0 = (hi[0] - lo[0] + TL::E0 * TL::CH - 1) / (TL::E0 * TL::CH);
-      t1 = (int)tile / (n0 * n2);
-      const int rem = (int)tile - t1 * n0 * n2;
-      t0 = rem / n2;
-      t2 = rem - t0 * n2;
+      /* Row bands in groups of G; inside a group plane chunks slowest, then
+       * bands, then column tiles.  A chunk's halo planes were read by the
+       * group's previous chunk G x 64 x 4 x (row bytes) earlier (512 KB at
+       * 512^3 FP32), inside the 4 MB L2 of the XCD the group runs on, and a
+       * band's halo rows by the band before it, n2 tiles earlier.  With row
+       * bands slowest, the reuse distance of a halo row was a whole band
+       * (4 MB there): 282 against 312 us with aligned columns. */
+      constexpr int64_t G = 16;
+      const int64_t n2 = (hi[2] - a2 + TL::E2 - 1) / TL::E2;
+      const int64_t n1 = (hi[1] - lo[1] + TL::E1 - 1) / TL::E1;
+      const int64_t n0 = (hi[0] - lo[0] + TL::E0 * TL::CH - 1) / (TL::E0 * TL::CH);
+      const int64_t per = G * n0 * n2;
+      const int64_t g = (int64_t)tile / per;
+      const int64_t gsz = min(G, n1 - g * G);
+      const int64_t rem = (int64_t)tile - g * per;
+      t0 = (int)(rem / (gsz * n2));
+      const int64_t r2 = rem - (int64_t)t0 * gsz * n2;
+      const int64_t b = r2 / n2;
+      t1 = (int)(g * G + b);
+      t2 = (int)(r2 - b * n2);
     }
     tlo[0] = lo[0] + t0 * (TL::E0 * TL::CH) + chunk * TL::E0;
     tlo[1] = D > 1 ? lo[1] + t1 * TL::E1 : 0;
-    tlo[2] = D > 2 ? lo[2] + t2 * TL::E2 : 0;
+    const int c2 = a2 + t2 * TL::E2;  /* first column of the (aligned) 3-D tile */
+    tlo[2] = D > 2 ? max(c2, lo[2]) : 0;
     thi[0] = min(tlo[0] + TL::E0, hi[0]);
     thi[1] = D > 1 ? min(tlo[1] + TL::E1, hi[1]) : 1;
-    thi[2] = D > 2 ? min(tlo[2] + TL::E2, hi[2]) : 1;
+    thi[2] = D > 2 ? min(c2 + TL::E2, hi[2]) : 1;
 #pragma unroll
     for (int n = 0; n < NC; ++n) {
       if (D == 1) {
@@ -217,19 +238,20 @@ struct TileCells {
       } else {
         c[n][0] = tlo[0] + n;
         c[n][1] = tlo[1] + (tid >> 6);
-        c[n][2] = tlo[2] + (tid & 63);
+        c[n][2] = c2 + (tid & 63);
       }
-      in[n] = c[n][0] < thi[0] && (D < 2 || c[n][1] < thi[1]) && (D < 3 || c[n][2] < thi[2]);
+      in[n] = c[n][0] < thi[0] && (D < 2 || c[n][1] < thi[1]) && (D < 3 || (c[n][2] >= tlo[2] && c[n][2] < thi[2]));
     }
   }
 
-  /* Move cells past the tile's edge onto its last cell (in[] keeps which are real). */
+  /* Move cells outside the tile's clipped extent onto its nearest cell (in[]
+   * keeps which are real). */
   __device__ __forceinline__ void clamp_to_tile() {
 #pragma unroll
     for (int n = 0; n < NC; ++n) {
       c[n][0] = min(c[n][0], thi[0] - 1);
       if (D > 1) c[n][1] = min(c[n][1], thi[1] - 1);
-      if (D > 2) c[n][2] = min(c[n][2], thi[2] - 1);
+      if (D > 2) c[n][2] = max(min(c[n][2], thi[2] - 1), tlo[2]);
     }
   }
 };

@@ -197,7 +197,8 @@ int fill_refs(xg_kernel* k, const xg_stmt_desc& d, std::vector<xg_gview>& refs) 
 int64_t tiles_for(int D, const int32_t* lo, const int32_t* hi) {
   if (D == 1) return ceil_div(hi[0] - lo[0], 1024);
   if (D == 2) return ceil_div(hi[0] - lo[0], 4) * ceil_div(hi[1] - lo[1], 64);
-  return ceil_div(hi[0] - lo[0], 4 * XG_TILE3_CHUNKS) * ceil_div(hi[1] - lo[1], 4) * ceil_div(hi[2] - lo[2], 64);
+  // column tiles start at multiples of 64 columns (TileCells in xgrid_device.h)
+  return ceil_div(hi[0] - lo[0], 4 * XG_TILE3_CHUNKS) * ceil_div(hi[1] - lo[1], 4) * ceil_div(hi[2] - (lo[2] & ~63), 64);
 }
 
 int64_t blocks_for_dom(int D, const xg_dom& dom) {
