@@ -1,6 +1,7 @@
 """Frontend semantics (no GPU): the rules of xgrid/lang/parser.py and the
 stencil analysis of xgrid/lang/generator.py:50-76."""
 import dataclasses
+import os
 
 import numpy as np
 import pytest
@@ -190,3 +191,102 @@ def test_fill_default_time_truncates_and_raises(double):
     g2 = xgrid.Grid((4,), float)
     g2.fill(np.full(4, 3.0), time=-1)
     assert (g2.now == 3.0).all()
+
+
+def _stmt_struct(src: str, sid: int) -> str:
+    i = src.index(f"struct xg_s{sid} ")
+    return src[i:src.index("extern", i)]
+
+
+def test_staged_3d_plan(double, monkeypatch):
+    """3-D statements whose loads of 8-byte arrays include column or row
+    neighbours are staged: plane registers, DPP column shifts, an LDS copy of
+    the tile's rows (only for row neighbours); diagonal loads stay global
+    loads; 2-D statements, 4-byte arrays and XGRID_STAGE3=0 keep plain loads."""
+    from xgrid_amd import workloads as W
+    from xgrid_amd.backend import lower_kernel
+    f3 = xgrid.grid[float, 3]
+
+    src = lower_kernel(W.make_heat_3d())[0].device_src
+    s0 = _stmt_struct(src, 0)
+    assert "ST3 = true" in s0 and "ST3_LDS = 12288" in s0           # 4 planes x 6 rows x 64 columns x 8 B
+    assert "xg::ld<" not in s0[s0.index("value_st3"):]              # every load of the 7-point update staged
+    assert "amdgpu_waves_per_eu(8)))" in src[src.index("xg_k0(") - 80:src.index("xg_k0(")]
+
+    @xgrid.kernel()
+    def diag(u: f3, v: f3) -> None:
+        u[0, 0, 0] = v[0, 1, 1] + v[1, -1, 1]
+        with xgrid.boundary(1):
+            u[0, 0, 0] = v[0, 0, 1] * 2.0 - v[1, 0, 0]
+
+    src = lower_kernel(diag)[0].device_src
+    assert "ST3 = false" in _stmt_struct(src, 0)
+    s1 = _stmt_struct(src, 1)
+    assert "ST3 = true" in s1 and "ST3_LDS = 16;" in s1 and "__syncthreads" not in s1
+
+    assert "ST3 = false" in _stmt_struct(lower_kernel(W.make_diffusion_2d())[0].device_src, 0)
+    monkeypatch.setenv("XGRID_STAGE3", "0")
+    assert "ST3 = true" not in lower_kernel(W.make_heat_3d())[0].device_src
+    monkeypatch.delenv("XGRID_STAGE3")
+    xgrid.init(precision="float")                                     # 4-byte arrays: plain loads
+    assert "ST3 = true" not in lower_kernel(W.make_heat_3d())[0].device_src
+
+
+def _dpp_in_masked_regions(asm: str) -> dict:
+    """Per kernel of an AMD assembly listing: (DPP moves, DPP moves between an
+    s_and_saveexec and the s_or_b64 exec that ends its region)."""
+    import re
+    cur, depth, out = None, 0, {}
+    for line in asm.splitlines():
+        m = re.match(r"^([A-Za-z_]\w*):", line)
+        if m:
+            cur, depth = m.group(1), 0
+            continue
+        t = line.strip()
+        if cur is None:
+            continue
+        if t.startswith("s_and_saveexec"):
+            depth += 1
+        elif t.startswith("s_or_b64 exec, exec,"):
+            depth = max(0, depth - 1)
+        if "_dpp" in t:
+            n, bad = out.get(cur, (0, 0))
+            out[cur] = (n + 1, bad + (depth > 0))
+    return out
+
+
+def test_staged_lane_shifts_run_on_every_lane(double):
+    """A lane shift reads another lane's register; run by a subset of lanes it
+    reads 0 from the disabled ones.  Compile the staged 3-D kernels for gfx950
+    (same options as at run time) and check that no DPP move of theirs sits in
+    an exec-masked region."""
+    import shutil
+    import subprocess
+    import tempfile
+    hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+    if not os.path.exists(hipcc):
+        pytest.skip("hipcc not available")
+    from xgrid_amd import compilers, workloads as W
+    from xgrid_amd.backend import lower_kernel
+    f3 = xgrid.grid[float, 3]
+
+    @xgrid.kernel()
+    def mixed(u: f3, v: f3) -> None:
+        u[0, 0, 0] = v[0, 0, 1] + v[-1, 0, 0] * 0.5 + v[0, 1, 0] * 0.25
+        with xgrid.boundary(4):
+            u[0, 0, 0] = u[0, 0, 1][0] + v[0, -1, 1] - v[0, 0, -1]
+
+    for op in (W.make_heat_3d(), mixed):
+        src = lower_kernel(op)[0].device_src
+        with tempfile.TemporaryDirectory() as d:
+            p = os.path.join(d, "k.hip")
+            with open(p, "w") as f:
+                f.write(src)
+            r = subprocess.run([hipcc, "-x", "hip", "--cuda-device-only", "-S", *compilers.device_options(),
+                                "-I", compilers.CSRC, p, "-o", os.path.join(d, "k.s")],
+                               capture_output=True, text=True, timeout=300)
+            assert r.returncode == 0, r.stderr[-2000:]
+            asm = open(os.path.join(d, "k.s")).read()
+        found = _dpp_in_masked_regions(asm)
+        assert found, "no staged kernel with lane shifts"
+        assert all(bad == 0 for _, bad in found.values()), found

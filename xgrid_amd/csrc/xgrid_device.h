@@ -108,6 +108,11 @@ __device__ __forceinline__ int64_t index(const xg_gview& g, int i0, int i1, int 
   return lin<D>(g.shape, i0, i1, i2);
 }
 
+/* Plane p clamped to the array's planes (staged 3-D loads of padding planes). */
+__device__ __forceinline__ int64_t st3_plane(int p, const int32_t* sh) {
+  return (int64_t)min(max(p, 0), sh[0] - 1);
+}
+
 /* Linear offset of a stencil offset under the strides of `shape`. */
 template <int D>
 __device__ __forceinline__ int64_t off(const int32_t* shape, int o0, int o1, int o2) {
@@ -180,6 +185,7 @@ struct TileCells {
   int c[NC][3];
   bool in[NC];
   int32_t tlo[3], thi[3];
+  int32_t col0;  /* 3-D: first column of the aligned tile (work-item c holds column col0 + c % 64) */
 
   __device__ __forceinline__ TileCells(const int32_t* lo, const int32_t* hi, uint32_t tile, int chunk = 0) {
     typedef Tile<D> TL;
@@ -221,6 +227,7 @@ struct TileCells {
     tlo[0] = lo[0] + t0 * (TL::E0 * TL::CH) + chunk * TL::E0;
     tlo[1] = D > 1 ? lo[1] + t1 * TL::E1 : 0;
     const int c2 = a2 + t2 * TL::E2;  /* first column of the (aligned) 3-D tile */
+    col0 = c2;
     tlo[2] = D > 2 ? max(c2, lo[2]) : 0;
     thi[0] = min(tlo[0] + TL::E0, hi[0]);
     thi[1] = D > 1 ? min(tlo[1] + TL::E1, hi[1]) : 1;
@@ -270,7 +277,7 @@ __device__ __forceinline__ bool tile_safe(const xg_launch_hdr& H, const int32_t*
 
 template <class S, int OV, bool MASKED>
 __device__ __forceinline__ void box_cells(const xg_launch_hdr& H, const typename S::Refs& R,
-                                          const typename S::Params& P, uint32_t tile) {
+                                          const typename S::Params& P, uint32_t tile, unsigned char* st3_lds) {
   constexpr int D = S::D;
   typedef typename S::T T;
   for (int ch = 0; ch < Tile<D>::CH; ++ch) {
@@ -288,8 +295,24 @@ __device__ __forceinline__ void box_cells(const xg_launch_hdr& H, const typename
     }
     T v[NC];
     if (tile_safe<S>(H, tc.tlo, tc.thi)) {
+      if constexpr (D == 3 && S::ST3) {
+        /* Staged 3-D tile: each array's columns of the tile's planes are read
+         * once into registers; column neighbours come over DPP lane shifts and
+         * row neighbours from an LDS copy of the tile's rows.  tile_safe is the
+         * same for every work-item of the workgroup, so all of them reach the
+         * barrier inside st3_load. */
+        typename S::St3 st;
+        S::st3_load(st, st3_lds, R, H.out.shape, tc.tlo[0], tc.tlo[1], tc.col0);
 #pragma unroll
-      for (int n = 0; n < NC; ++n) v[n] = S::template value<true>(R, P, H.out.shape, tc.c[n][0], tc.c[n][1], tc.c[n][2]);
+        for (int n = 0; n < NC; ++n) {
+          S::st3_plane_n(st, n);
+          v[n] = S::value_st3(st, R, P, H.out.shape, n, tc.c[n][0], tc.c[n][1], tc.c[n][2]);
+        }
+        if (Tile<D>::CH > 1) __syncthreads();  /* the next chunk rewrites the LDS copy */
+      } else {
+#pragma unroll
+        for (int n = 0; n < NC; ++n) v[n] = S::template value<true>(R, P, H.out.shape, tc.c[n][0], tc.c[n][1], tc.c[n][2]);
+      }
     } else {
 #pragma unroll
       for (int n = 0; n < NC; ++n) v[n] = S::template value<false>(R, P, H.out.shape, tc.c[n][0], tc.c[n][1], tc.c[n][2]);
@@ -314,10 +337,15 @@ __device__ void run_main(const xg_launch_hdr& H, const typename S::Refs& R, cons
       unlin<D>(H.out.shape, gp<const int32_t>(dom.list)[k], i0, i1, i2);
       apply<S, OV>(H, R, P, i0, i1, i2);
     }
+  } else if constexpr (D == 3 && S::ST3) {
+    /* one LDS copy for both box variants (staged 3-D loads) */
+    __shared__ __attribute__((aligned(16))) unsigned char st3_lds[S::ST3_LDS];
+    if (dom.kind == XG_DOM_MASKED) box_cells<S, OV, true>(H, R, P, tile, st3_lds);
+    else box_cells<S, OV, false>(H, R, P, tile, st3_lds);
   } else if (dom.kind == XG_DOM_MASKED) {
-    box_cells<S, OV, true>(H, R, P, tile);
+    box_cells<S, OV, true>(H, R, P, tile, nullptr);
   } else {
-    box_cells<S, OV, false>(H, R, P, tile);
+    box_cells<S, OV, false>(H, R, P, tile, nullptr);
   }
 }
 

@@ -201,6 +201,179 @@ def function_symbol(op) -> str:
     return "xgf_" + op.c_name
 
 
+class _St3Plan:
+    """Staged loads of a 3-D statement over one tile (4 planes x 4 rows x 64
+    columns, work-item = (row wave, column lane), 4 planes per work-item), for
+    arrays of 8-byte elements.
+
+    Per array j (a distinct (grid, level) of the statement) the columns of the
+    planes the statement reads are held in registers `q{j}[u]`, plane
+    p0 + pmin + u.  A load at offset (o0, 0, 0) reads q{j}[n + o0 - pmin]; at
+    (o0, 0, +-1) the neighbouring lane's value, moved with a DPP lane shift
+    (lanes 0 and 63 read their outer neighbour from memory); at (o0, +-1, 0)
+    a row of an LDS copy of the tile's 4 rows plus one halo row on each side.
+    Other loads (diagonals, larger offsets, lower-dimensional grids) stay
+    global loads.  Only used in tiles whose cells and reach lie inside the grid,
+    so every value a stored cell uses is exact; padding lanes and rows read
+    clamped addresses and are not stored."""
+
+    NP, ROWS, COLS = 4, 4, 64
+    MAX_WINDOW = 8          # most planes held in registers per array
+    MAX_LDS_GROUPS = 3      # most (array, plane offset) pairs staged in LDS
+
+    def __init__(self, refs, types):
+        self.refs = refs            # [(grid, level)]
+        self.types = types          # device element type per ref index
+        self.planes = {}            # j -> set of o0 read by staged loads
+        self.xg = []                # [(j, o0)] with a column-neighbour load
+        self.xside = {}             # (j, o0) -> set of +-1
+        self.yg = []                # [(j, o0)] with a row-neighbour load
+
+    def add(self, j, off) -> bool:
+        o0, o1, o2 = off
+        if self._size(j) != 8:
+            # 4-byte arrays stay global loads: a wave's row segment is then two cache
+            # lines, and the plain loads beat the LDS copy and barrier (512^3 heat in
+            # FP32: 0.29 ms per step plain, 0.37 ms staged; FP64: 0.74 and 0.48 ms)
+            return False
+        if o1 == 0 and o2 == 0:
+            self.planes.setdefault(j, set()).add(o0)
+        elif o1 == 0 and abs(o2) == 1:
+            self.planes.setdefault(j, set()).add(o0)
+            if (j, o0) not in self.xg:
+                self.xg.append((j, o0))
+            self.xside.setdefault((j, o0), set()).add(o2)
+        elif abs(o1) == 1 and o2 == 0:
+            self.planes.setdefault(j, set()).add(o0)
+            if (j, o0) not in self.yg:
+                self.yg.append((j, o0))
+        else:
+            return False
+        return True
+
+    def usable(self) -> bool:
+        if not self.xg and not self.yg:
+            return False            # nothing a plain load would not do as well
+        if len(self.yg) > self.MAX_LDS_GROUPS:
+            return False
+        return all(max(p) - min(p) + self.NP <= self.MAX_WINDOW for p in self.planes.values())
+
+    def pmin(self, j) -> int:
+        return min(self.planes[j])
+
+    def window(self, j) -> int:
+        return max(self.planes[j]) - self.pmin(j) + self.NP
+
+    def access(self, j, off):
+        if j not in self.planes:
+            return None
+        o0, o1, o2 = off
+        if o1 == 0 and o2 == 0:
+            return f"st.q{j}[n + {o0 - self.pmin(j)}]"
+        if o1 == 0 and abs(o2) == 1 and (j, o0) in self.xg:
+            return f"st.x{'p' if o2 > 0 else 'n'}{self.xg.index((j, o0))}"
+        if abs(o1) == 1 and o2 == 0 and (j, o0) in self.yg:
+            return f"st.y{'p' if o1 > 0 else 'n'}{self.yg.index((j, o0))}"
+        return None
+
+    def lds_bytes(self) -> int:
+        rows = self.ROWS + 2
+        return sum(self.NP * rows * self.COLS * self._size(j) for j, _ in self.yg)
+
+    def _size(self, j) -> int:
+        return 8 if self.types[j] in ("double", "int64_t", "uint64_t") else 4
+
+    def members(self, expr: str) -> list:
+        """st3_load: the plane registers, the edge lanes' outer columns and the
+        LDS copy (ends with a barrier); st3_plane_n: the neighbours of plane
+        n, formed just before its value so that only one plane's are live;
+        value_st3: the statement's expression over them."""
+        NP, R2, C = self.NP, self.ROWS + 2, self.COLS
+        out = ["  static constexpr bool ST3 = true;", f"  static constexpr int ST3_LDS = {max(16, self.lds_bytes())};"]
+        fields = []
+        for j in sorted(self.planes):
+            fields.append(f"{self.types[j]} q{j}[{self.window(j)}];")
+        for g, (j, o0) in enumerate(self.xg):
+            fields.append(f"{self.types[j]} e{g}[{NP}];")
+            for sd in sorted(self.xside[(j, o0)]):
+                fields.append(f"{self.types[j]} x{'p' if sd > 0 else 'n'}{g};")
+        for g, (j, o0) in enumerate(self.yg):
+            fields.append(f"{self.types[j]} yp{g}; {self.types[j]} yn{g};")
+        fields.append("const unsigned char* lds; int lane, wr;")
+        out.append(f"  struct St3 {{ {' '.join(fields)} }};")
+        out.append("  __device__ static __forceinline__ void st3_load(St3& st, unsigned char* lds, const Refs& R, "
+                   "const int32_t* sh, int p0, int r0, int c0) {")
+        out.append("    const int lane = threadIdx.x & 63, wr = threadIdx.x >> 6;")
+        out.append("    st.lds = lds; st.lane = lane; st.wr = wr;")
+        out.append("    const int64_t P1 = sh[2], P0 = (int64_t)sh[1] * sh[2];")
+        out.append("    const int64_t row = min(r0 + wr, sh[1] - 1), col = min(c0 + lane, sh[2] - 1);")
+        out.append("    (void)lds; (void)P1; (void)P0;")
+        for j in sorted(self.planes):
+            t = self.types[j]
+            out.append(f"    const xg::gptr<const {t}> g{j} = xg::gp<const {t}>(R.v[{j}].ptr);")
+            out.append("#pragma unroll")
+            out.append(f"    for (int u = 0; u < {self.window(j)}; ++u) "
+                       f"st.q{j}[u] = g{j}[xg::st3_plane(p0 + u + {self.pmin(j)}, sh) * P0 + row * P1 + col];")
+        if self.xg:
+            out.append("    const bool edge = lane == 0 || lane == 63;")
+            out.append("    const int64_t ecol = lane == 0 ? max(c0 - 1, 0) : min(c0 + 64, sh[2] - 1);")
+        for g, (j, o0) in enumerate(self.xg):
+            t = self.types[j]
+            out.append("#pragma unroll")
+            out.append(f"    for (int n = 0; n < {NP}; ++n) st.e{g}[n] = edge ? "
+                       f"g{j}[xg::st3_plane(p0 + n + {o0}, sh) * P0 + row * P1 + ecol] : ({t})0;")
+        off = 0
+        self._offs = []
+        for g, (j, o0) in enumerate(self.yg):
+            t = self.types[j]
+            self._offs.append(off)
+            out.append(f"    {{ {t}* L = ({t}*)(lds + {off});")
+            out.append("#pragma unroll")
+            out.append(f"      for (int n = 0; n < {NP}; ++n) L[(n * {R2} + wr + 1) * {C} + lane] = "
+                       f"st.q{j}[n + {o0 - self.pmin(j)}];")
+            out.append("#pragma unroll")
+            out.append("      for (int h = 0; h < 2; ++h) {")
+            out.append(f"        const int e = threadIdx.x + h * 256, hr = e >> 8, pl = (e >> 6) & {NP - 1};")
+            out.append(f"        const int64_t gr = min(max(hr ? r0 + {self.ROWS} : r0 - 1, 0), sh[1] - 1);")
+            out.append(f"        L[(pl * {R2} + (hr ? {R2 - 1} : 0)) * {C} + lane] = "
+                       f"g{j}[xg::st3_plane(p0 + pl + {o0}, sh) * P0 + gr * P1 + col];")
+            out.append("      }")
+            out.append("    }")
+            off += NP * R2 * C * self._size(j)
+        if self.yg:
+            out.append("    __syncthreads();")
+        out.append("  }")
+        # Every work-item of the workgroup runs this for every n, and the lane shifts come
+        # before the edge lanes' selects: a shift written inside `lane == 63 ? e : shift`
+        # is evaluated only by the other lanes, and a lane reading a disabled lane gets 0.
+        out.append("  __device__ static __forceinline__ void st3_plane_n(St3& st, int n) {")
+        out.append("    (void)st; (void)n;")
+        for g, (j, o0) in enumerate(self.xg):
+            t = self.types[j]
+            out.append(f"    {{ const {t} c = st.q{j}[n + {o0 - self.pmin(j)}];")
+            if 1 in self.xside[(j, o0)]:
+                out.append(f"      const {t} a = xg::lane_shift<1>(c);")
+                out.append(f"      st.xp{g} = st.lane == 63 ? st.e{g}[n] : a;")
+            if -1 in self.xside[(j, o0)]:
+                out.append(f"      const {t} b = xg::lane_shift<-1>(c);")
+                out.append(f"      st.xn{g} = st.lane == 0 ? st.e{g}[n] : b;")
+            out.append("    }")
+        for g, (j, o0) in enumerate(self.yg):
+            t = self.types[j]
+            out.append(f"    {{ const {t}* L = (const {t}*)(st.lds + {self._offs[g]});")
+            out.append(f"      st.yn{g} = L[(n * {R2} + st.wr) * {C} + st.lane]; "
+                       f"st.yp{g} = L[(n * {R2} + st.wr + 2) * {C} + st.lane]; }}")
+        out.append("  }")
+        out.append("  __device__ static __forceinline__ T value_st3(const St3& st, const Refs& R, const Params& P, "
+                   "const int32_t* sh, int n, int i0, int i1, int i2) {")
+        out.append("    constexpr bool SAFE = true;")
+        out.append("    const int64_t xc = xg::lin<3>(sh, i0, i1, i2);")
+        out.append("    (void)st; (void)R; (void)P; (void)sh; (void)n; (void)xc; (void)i0; (void)i1; (void)i2;")
+        out.append(f"    return {expr};")
+        out.append("  }")
+        return out
+
+
 class Emitter:
     def __init__(self, info: KernelInfo, overstep: str = "none", comment: bool = False,
                  entry: str = "xg_entry", macro: list | None = None, block_shape: tuple | None = None):
@@ -271,6 +444,18 @@ class Emitter:
             raise EmitError(f"{st.node.loc}: a statement may load at most 16 distinct (grid, time level) arrays")
         return refs
 
+    def _plan_st3(self, st: StencilStmt, refs: list, grids: dict):
+        """The staged plan of a 3-D statement, or None (plain loads).
+        XGRID_STAGE3=0 turns staging off (for comparisons)."""
+        if os.environ.get("XGRID_STAGE3", "1") == "0":
+            return None
+        types = [device_type_name(grids[g].element) for g, _ in refs]
+        plan = _St3Plan(refs, types)
+        for n in ir.walk(st.node.value):
+            if isinstance(n, ir.Cell) and grids[n.grid.name].dimension == 3:
+                plan.add(refs.index((n.grid.name, n.level)), tuple(n.offsets))
+        return plan if plan.usable() else None
+
     def _device_stmt(self, st: StencilStmt, refs: list, params: list, out: list, tb: LoopMeta | None = None):
         sid = st.sid
         D = st.dimension
@@ -281,8 +466,13 @@ class Emitter:
         pre_mode = [False]   # ... and the other loads come from a per-cell prefetch
         pre_slots: list = []  # (grid, level, offsets) of each prefetched load
         rd_slots: list = []   # distinct (row, column) offsets of the loop grid's level-0 reads
+        st3 = [None]          # while lowering value_st3: the staged 3-D plan (see _plan_st3)
 
         def cell(e: ir.Cell) -> str:
+            if st3[0] is not None:
+                acc = st3[0].access(refs.index((e.grid.name, e.level)), tuple(e.offsets))
+                if acc is not None:
+                    return acc
             if in_lds[0] and e.grid.name == st.grid and e.level == 0:
                 return f"xg::tb_ld<T, CHECK, RHH, RWW>(lds, vld, ok, lr + {e.offsets[0]}, lc + {e.offsets[1]})"
             if in_reg[0] and e.grid.name == st.grid and e.level == 0:
@@ -312,6 +502,12 @@ class Emitter:
             return f"R.v[{r}].shape[{_ExprC(True, cell, name, grid_info)(e.dim)}]"
 
         expr = _ExprC(True, cell, name, grid_info)(st.node.value)
+        expr_st3 = None
+        plan3 = self._plan_st3(st, refs, grids) if D == 3 and tb is None else None
+        if plan3 is not None:
+            st3[0] = plan3
+            expr_st3 = _ExprC(True, cell, name, grid_info)(st.node.value)
+            st3[0] = None
         expr_tb = expr_pre = expr_lds = expr_lds_pre = None
         if tb is not None:
             in_reg[0] = True
@@ -359,6 +555,11 @@ class Emitter:
         out.append("    (void)R; (void)P; (void)sh; (void)xc; (void)i0; (void)i1; (void)i2;")
         out.append(f"    return {expr};")
         out.append("  }")
+        if expr_st3 is not None:
+            out.extend(plan3.members(expr_st3))
+        else:
+            out.append("  static constexpr bool ST3 = false;")
+            out.append("  static constexpr int ST3_LDS = 0;")
         if expr_tb is not None:
             # the same expression with the loop grid's level-0 reads taken from s[], the values
             # at the statement's distinct offsets (gathered from registers by the blocked kernel)
@@ -412,7 +613,17 @@ class Emitter:
         out.append("};")
         args = f"const xg_launch_hdr H, const xg_r{sid} R, const xg_p{sid} P"
         remap = "xg::xcd_remap(blockIdx.x, gridDim.x)"
-        out.append(f'extern "C" __global__ void __launch_bounds__(256) xg_k{sid}({args}) '
+        # A staged 3-D statement's kernel holds plane registers and an LDS copy: ask for the
+        # occupancy its LDS allows (at most 8 waves per SIMD), which the register allocator
+        # then meets (the FP64 7-point update: 62 VGPRs instead of 100, no spills).
+        # The batched variant keeps more state per work-item (its entry copied into locals):
+        # two waves fewer keep it free of spills.
+        wv = wvb = ""
+        if expr_st3 is not None:
+            w = max(1, min(8, 163840 // max(1, plan3.lds_bytes())))
+            wv = f" __attribute__((amdgpu_waves_per_eu({w})))"
+            wvb = f" __attribute__((amdgpu_waves_per_eu({max(1, w - 2)})))"
+        out.append(f'extern "C" __global__ void __launch_bounds__(256){wv} xg_k{sid}({args}) '
                    f"{{ xg::run_main<xg_s{sid}, XG_OVERSTEP>(H, R, P, {remap}, gridDim.x); }}")
         # batched: one launch over the tiles of many instances (entries in a device table)
         # The entry's header, views and parameters are copied into locals first: read
@@ -421,7 +632,7 @@ class Emitter:
         bargs = "const xg_bent* __restrict__ E, const int32_t n, const int64_t total"
         local = (f"const xg_launch_hdr h = e.hdr; const xg_r{sid} r = *xg::cp<xg_r{sid}>(e.refs); "
                  f"const xg_p{sid} p = *xg::cp<xg_p{sid}>(e.params);")
-        out.append(f'extern "C" __global__ void __launch_bounds__(256) xg_k{sid}_b({bargs}) '
+        out.append(f'extern "C" __global__ void __launch_bounds__(256){wvb} xg_k{sid}_b({bargs}) '
                    f"{{ xg::for_batch_tiles(E, n, total, [&](const xg_bent& e, uint32_t t) "
                    f"{{ {local} xg::run_main<xg_s{sid}, XG_OVERSTEP>(h, r, p, t, e.ntiles); }}); }}")
         if st.mode == "pingpong":
