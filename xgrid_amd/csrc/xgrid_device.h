@@ -164,6 +164,24 @@ __device__ __forceinline__ void apply(const xg_launch_hdr& H, const typename S::
   store<S, OV>(H, i0, i1, i2, S::template value<false>(R, P, H.out.shape, i0, i1, i2));
 }
 
+/* Coordinates of list entry c.  Lists exist only for grids of at most
+ * INT32_MAX cells (the planner uses a masked box otherwise), so the entry and
+ * the plane size fit in 32 bits and the divisions are 32-bit ones (a 64-bit
+ * division is a long instruction sequence per cell). */
+template <int D>
+__device__ __forceinline__ void unlin_list(const int32_t* shape, int32_t c, int& i0, int& i1, int& i2) {
+  const uint32_t u = (uint32_t)c;
+  if (D == 1) { i0 = (int)u; i1 = 0; i2 = 0; }
+  else if (D == 2) {
+    const uint32_t s1 = (uint32_t)shape[1], q = u / s1;
+    i0 = (int)q; i1 = (int)(u - q * s1); i2 = 0;
+  } else {
+    const uint32_t s2 = (uint32_t)shape[2], plane = (uint32_t)shape[1] * s2;
+    const uint32_t q0 = u / plane, r = u - q0 * plane, q1 = r / s2;
+    i0 = (int)q0; i1 = (int)q1; i2 = (int)(r - q1 * s2);
+  }
+}
+
 template <int D>
 __device__ __forceinline__ void unlin(const int32_t* shape, int64_t c, int& i0, int& i1, int& i2) {
   if (D == 1) { i0 = (int)c; i1 = 0; i2 = 0; }
@@ -334,7 +352,7 @@ __device__ void run_main(const xg_launch_hdr& H, const typename S::Refs& R, cons
     const int64_t stride = ntiles * blockDim.x;
     for (int64_t k = (int64_t)tile * blockDim.x + threadIdx.x; k < dom.count; k += stride) {
       int i0, i1, i2;
-      unlin<D>(H.out.shape, gp<const int32_t>(dom.list)[k], i0, i1, i2);
+      unlin_list<D>(H.out.shape, gp<const int32_t>(dom.list)[k], i0, i1, i2);
       apply<S, OV>(H, R, P, i0, i1, i2);
     }
   } else if constexpr (D == 3 && S::ST3) {
@@ -426,7 +444,7 @@ __device__ void run_phase2(const xg_launch_hdr& H, uint32_t tile, int64_t ntiles
     const int64_t stride = ntiles * blockDim.x;
     for (int64_t k = (int64_t)tile * blockDim.x + threadIdx.x; k < dom.count; k += stride) {
       int i0, i1, i2;
-      unlin<D>(shape, gp<const int32_t>(dom.list)[k], i0, i1, i2);
+      unlin_list<D>(shape, gp<const int32_t>(dom.list)[k], i0, i1, i2);
       f(i0, i1, i2);
     }
   } else if (dom.kind == XG_DOM_MASKED) {
@@ -448,7 +466,7 @@ __device__ void run_block(const xg_launch_hdr& H, const typename S::Refs& R, con
   if (dom.kind == XG_DOM_LIST) {
     for (int64_t k = threadIdx.x; k < dom.count; k += blockDim.x) {
       int i0, i1, i2;
-      unlin<D>(H.out.shape, gp<const int32_t>(dom.list)[k], i0, i1, i2);
+      unlin_list<D>(H.out.shape, gp<const int32_t>(dom.list)[k], i0, i1, i2);
       apply<S, OV>(H, R, P, i0, i1, i2);
     }
     return;

@@ -16,6 +16,7 @@
 #include <string>
 #include <unordered_map>
 #include <utility>
+#include <climits>
 #include <vector>
 
 #include "xgrid_internal.h"
@@ -977,8 +978,10 @@ int xg_grid_set_mask(xg_grid* g, const int32_t* mask) {
       p.hi[d] = kv.second.hi[d] + 1;
       vol *= (p.hi[d] - p.lo[d]);
     }
+    // A list holds int32 linear indices: a grid of more than INT32_MAX cells plans its
+    // sparse classes as masked boxes instead (the device unpacks list entries in 32 bits).
     if (p.count == vol) p.kind = XG_DOM_BOX;
-    else if (p.count * 4 >= vol) p.kind = XG_DOM_MASKED;
+    else if (p.count * 4 >= vol || g->n > (int64_t)INT32_MAX) p.kind = XG_DOM_MASKED;
     else {
       p.kind = XG_DOM_LIST;
       lists[kv.first].reserve((size_t)p.count);
