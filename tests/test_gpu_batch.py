@@ -104,6 +104,39 @@ def test_batched_diffusion_2d_equal_alone(double):
     batch.close()
 
 
+@pytest.mark.parametrize("prec", ["double", "float"])
+def test_batched_heat_3d_equal_alone(prec, request):
+    """Batched 3-D statements (staged loads in FP64, plain loads in FP32) on
+    non-cubic grids, one narrower than a 64-column tile, equal the same
+    instances run alone."""
+    xg = request.getfixturevalue("double" if prec == "double" else "single")
+    from xgrid_amd import workloads as W
+    from xgrid_amd.batch import Batch
+    kb, kp = W.make_heat_3d(), W.make_heat_3d()
+    batch = Batch(kb)
+    rng = np.random.default_rng(5)
+    dt_np = np.float64 if prec == "double" else np.float32
+    pairs = []
+    for shape in ((34, 34, 34), (20, 45, 150), (66, 9, 130), (20, 30, 40)):
+        u0 = rng.random(shape).astype(dt_np)
+        nu, dx = 0.1, 1.0 / (max(shape) - 1)
+        dt = 0.1 * dx * dx / nu
+        gs = []
+        for _ in range(2):
+            g = xg.Grid(shape, float)
+            g.now[...] = u0
+            g.boundary[...] = W.box_walls(shape)
+            gs.append(g)
+        batch.add(gs[0], nu, dt, dx)
+        pairs.append((gs, (nu, dt, dx)))
+    batch.step(7)
+    for gs, a in pairs:
+        for _ in range(7):
+            kp(gs[1], *a)
+        _assert_same([gs[0]], [gs[1]])
+    batch.close()
+
+
 def test_batch_partitions_instances_by_control_flow(double):
     """Instances whose calls take different paths (here: loop trip counts) are
     walked in separate groups and still match their plain runs."""
