@@ -60,6 +60,8 @@ typedef struct xg_stmt_desc {
   void* fn_main_b;                           /* batched variants (many instances per launch) */
   void* fn_pingpong_b;
   void* fn_phase2_b;
+  void* fn_pair;                             /* this statement and the next in one launch (both in
+                                                place over one box), or NULL */
 } xg_stmt_desc;
 
 /* A run of consecutive statements that may be chained in one workgroup. */

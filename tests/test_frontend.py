@@ -298,3 +298,23 @@ def test_staged_lane_shifts_run_on_every_lane(double):
         found = {k: v for k, v in _dpp_in_masked_regions(asm).items() if not k.endswith("_b")}
         assert found, "no staged kernel with lane shifts"
         assert all(bad == 0 for _, bad in found.values()), found
+
+
+def test_statement_pairs_marked(double):
+    """Consecutive in-place statements of one group storing to different grids,
+    neither loading the other's store grid at time 0, get a pair kernel."""
+    from xgrid_amd import workloads as W
+    from xgrid_amd.backend import lower_kernel
+    low = lower_kernel(W.make_cavity())[0]
+    pairs = {m.stmt.sid: m.pair for m in low.stmts if m.pair}
+    assert pairs.get(11) == "xg_f11"                       # u and v updates
+    assert 0 not in pairs                                  # the first sweep reads b at time 0
+    assert "run_pair<xg_s11, xg_s12, XG_OVERSTEP>" in low.device_src
+    f2 = xgrid.grid[float, 2]
+
+    @xgrid.kernel()
+    def chain(u: f2, v: f2) -> None:
+        u[0, 0] = v[0, 1]
+        v[0, 0] = u[0, 0][0] * 2.0                         # reads u at time 0: no pair
+
+    assert not any(m.pair for m in lower_kernel(chain)[0].stmts)

@@ -34,6 +34,7 @@ class StmtDesc(Structure):
         ("param_bytes", c_int32), ("overstep", c_int32), ("group", c_int32), ("group_pos", c_int32),
         ("fn_main", c_void_p), ("fn_pingpong", c_void_p), ("fn_phase2", c_void_p),
         ("fn_main_b", c_void_p), ("fn_pingpong_b", c_void_p), ("fn_phase2_b", c_void_p),
+        ("fn_pair", c_void_p),
     ]
 
 

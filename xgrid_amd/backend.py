@@ -116,6 +116,7 @@ class HipKernel:
             d.fn_main_b = fn(m.kernel + "_b")
             d.fn_pingpong_b = fn(m.kernel_pp + "b") if m.kernel_pp else None
             d.fn_phase2_b = fn(m.phase2 + "_b")
+            d.fn_pair = fn(m.pair) if m.pair else None
         groups = (_lib.GroupDesc * max(1, len(low.groups)))()
         for i, g in enumerate(low.groups):
             groups[i].first = g.first

@@ -367,6 +367,45 @@ __device__ void run_main(const xg_launch_hdr& H, const typename S::Refs& R, cons
   }
 }
 
+/* Two in-place statements over the same box in one tile walk (the runtime
+ * pairs them only when their classes are that same box and neither reads the
+ * other's store grid at time 0, so the order of their stores does not matter).
+ * Each work-item evaluates both statements at its cells, then stores both. */
+template <class SA, class SB, int OV>
+__device__ void run_pair(const xg_launch_hdr& HA, const typename SA::Refs& RA, const typename SA::Params& PA,
+                         const xg_launch_hdr& HB, const typename SB::Refs& RB, const typename SB::Params& PB,
+                         uint32_t tile) {
+  constexpr int D = SA::D;
+  static_assert(SB::D == D, "paired statements share one tile");
+  for (int ch = 0; ch < Tile<D>::CH; ++ch) {
+    TileCells<D> tc(HA.dom.lo, HA.dom.hi, tile, ch);
+    if (tc.tlo[0] >= HA.dom.hi[0]) break;
+    constexpr int NC = TileCells<D>::NC;
+    tc.clamp_to_tile();  /* evaluate every cell without branching (see box_cells) */
+    typename SA::T va[NC];
+    typename SB::T vb[NC];
+    if (tile_safe<SA>(HA, tc.tlo, tc.thi) && tile_safe<SB>(HB, tc.tlo, tc.thi)) {
+#pragma unroll
+      for (int n = 0; n < NC; ++n) {
+        va[n] = SA::template value<true>(RA, PA, HA.out.shape, tc.c[n][0], tc.c[n][1], tc.c[n][2]);
+        vb[n] = SB::template value<true>(RB, PB, HB.out.shape, tc.c[n][0], tc.c[n][1], tc.c[n][2]);
+      }
+    } else {
+#pragma unroll
+      for (int n = 0; n < NC; ++n) {
+        va[n] = SA::template value<false>(RA, PA, HA.out.shape, tc.c[n][0], tc.c[n][1], tc.c[n][2]);
+        vb[n] = SB::template value<false>(RB, PB, HB.out.shape, tc.c[n][0], tc.c[n][1], tc.c[n][2]);
+      }
+    }
+#pragma unroll
+    for (int n = 0; n < NC; ++n)
+      if (tc.in[n]) {
+        store<SA, OV>(HA, tc.c[n][0], tc.c[n][1], tc.c[n][2], va[n]);
+        store<SB, OV>(HB, tc.c[n][0], tc.c[n][1], tc.c[n][2], vb[n]);
+      }
+  }
+}
+
 /* Jacobi sweep: out = scratch, src = old level 0.  Every cell of the grid is
  * written exactly once, so after the sweep the two buffers swap roles. */
 template <class S, int OV>

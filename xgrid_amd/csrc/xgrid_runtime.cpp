@@ -312,6 +312,49 @@ int exec_group(xg_kernel* k, int32_t gi, const std::vector<int>& sids, const std
   return do_launch(k, -1, gd.fn, 1, kGroupThreads, args.data());
 }
 
+// Statements a and a + 1 run as one launch (the emitter's pair kernel) when both
+// run in place over the same box and no argument aliases a store grid the other
+// statement reads at time 0 or stores to.  A statement being timed runs alone.
+bool pairable(xg_kernel* k, int32_t a, int32_t b) {
+  if (b != a + 1 || b >= (int32_t)k->stmts.size()) return false;
+  StmtPlan& A = k->stmts[(size_t)a];
+  StmtPlan& B = k->stmts[(size_t)b];
+  if (!A.d.fn_pair || k->timed_sid == a || k->timed_sid == b) return false;
+  if (effective_mode(k, A) != XG_MODE_INPLACE || effective_mode(k, B) != XG_MODE_INPLACE) return false;
+  xg_grid* GA = k->grids[(size_t)A.d.store_arg];
+  xg_grid* GB = k->grids[(size_t)B.d.store_arg];
+  if (GA == GB) return false;
+  for (int r = 0; r < A.d.nrefs; ++r)
+    if (A.d.ref_level[r] == 0 && k->grids[(size_t)A.d.ref_arg[r]] == GB) return false;
+  for (int r = 0; r < B.d.nrefs; ++r)
+    if (B.d.ref_level[r] == 0 && k->grids[(size_t)B.d.ref_arg[r]] == GA) return false;
+  const ClassPlan& pa = GA->plan(A.d.klass);
+  const ClassPlan& pb = GB->plan(B.d.klass);
+  if (pa.kind != XG_DOM_BOX || pb.kind != XG_DOM_BOX || pa.count == 0) return false;
+  for (int d = 0; d < 3; ++d)
+    if (pa.lo[d] != pb.lo[d] || pa.hi[d] != pb.hi[d]) return false;
+  return true;
+}
+
+int exec_pair(xg_kernel* k, int32_t a, const void* pa, const void* pb) {
+  xg_launch_hdr H[2];
+  std::vector<xg_gview> refs[2];
+  for (int i = 0; i < 2; ++i) {
+    const xg_stmt_desc& d = k->stmts[(size_t)(a + i)].d;
+    xg_grid* G = k->grids[(size_t)d.store_arg];
+    std::memset(&H[i], 0, sizeof(xg_launch_hdr));
+    fill_dom(H[i].dom, G->plan(d.klass), G, d.klass);
+    int rc = fill_refs(k, d, refs[i]);
+    if (rc) return rc;
+    H[i].dom.flags = uniform_refs(k, d) ? XG_F_UNIFORM : 0;
+    fill_view(H[i].out, G, G->levels[0]);
+  }
+  void* args[6] = {&H[0], refs[0].data(), const_cast<void*>(pa), &H[1], refs[1].data(), const_cast<void*>(pb)};
+  k->n_fused += 2;
+  const xg_stmt_desc& d = k->stmts[(size_t)a].d;
+  return do_launch(k, -1, d.fn_pair, blocks_for_dom(d.dims, H[0].dom), kThreads, args);
+}
+
 int flush(xg_kernel* k) {
   if (k->pending.empty()) return XG_OK;
   std::vector<Pending> run;
@@ -337,6 +380,11 @@ int flush(xg_kernel* k) {
   size_t i = 0;
   while (i < run.size() && rc == XG_OK) {
     if (!ok[i]) {
+      if (i + 1 < run.size() && !ok[i + 1] && pairable(k, run[i].sid, run[i + 1].sid)) {
+        rc = exec_pair(k, run[i].sid, params_of(run[i]), params_of(run[i + 1]));
+        i += 2;
+        continue;
+      }
       rc = exec_stmt(k, run[i].sid, params_of(run[i]));
       ++i;
       continue;

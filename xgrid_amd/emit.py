@@ -82,6 +82,7 @@ class StmtMeta:
     phase2: str
     group: int = -1
     group_pos: int = 0
+    pair: str | None = None       # kernel running this statement and the next in one launch
 
 
 @dataclass
@@ -1079,6 +1080,14 @@ class Emitter:
             self._device_function(op, out)
         for m in stmt_meta:
             self._device_stmt(m.stmt, m.refs, m.params, out, loop_of.get(m.stmt.sid))
+        for m, nxt in zip(stmt_meta, stmt_meta[1:]):
+            if m.pair:
+                a, b = m.stmt.sid, nxt.stmt.sid
+                out.append(f'extern "C" __global__ void __launch_bounds__(256) {m.pair}('
+                           f"const xg_launch_hdr HA, const xg_r{a} RA, const xg_p{a} PA, "
+                           f"const xg_launch_hdr HB, const xg_r{b} RB, const xg_p{b} PB) "
+                           f"{{ xg::run_pair<xg_s{a}, xg_s{b}, XG_OVERSTEP>(HA, RA, PA, HB, RB, PB, "
+                           f"xg::xcd_remap(blockIdx.x, gridDim.x)); }}")
         for L in loops:
             for V in (L, L.alt):
                 if V is not None:
@@ -1337,6 +1346,7 @@ class Emitter:
                                       f"xg_k{st.sid}_pp" if st.mode == "pingpong" else None,
                                       phase2_names[key]))
         groups = self._groups(stmt_meta)
+        self._pairs(stmt_meta)
         loops = self._blockable_loops(stmt_meta)
         graphs_ok = not any(isinstance(n, ir.Inline) for n in ir.walk_all(self.func.body)) and \
             not any(op.mode == "external" for op in self.functions)
@@ -1436,6 +1446,27 @@ class Emitter:
             if (rows + 2 * halo) // waves > 16:
                 return None
         return LoopMeta(lid, node, ms, g, (rows, cols), halo, tmax, name, threads, reach, variant)
+
+    def _pairs(self, stmt_meta: list) -> None:
+        """Mark statement pairs that may run as one launch: consecutive in-place 1-D or
+        2-D statements of one group, storing unshifted to different grids, neither
+        loading the other's store grid at time 0 (the cavity's u and v updates).  The
+        runtime runs a pair together only when both classes are the same box and no
+        two arguments alias (`pairable` in xgrid_runtime.cpp)."""
+        i = 0
+        while i + 1 < len(stmt_meta):
+            a, b = stmt_meta[i], stmt_meta[i + 1]
+            sa, sb = a.stmt, b.stmt
+            ok = (a.group >= 0 and a.group == b.group and b.group_pos == a.group_pos + 1 and
+                  sa.mode == "inplace" and sb.mode == "inplace" and
+                  sa.dimension == sb.dimension and sa.dimension in (1, 2) and
+                  not any(sa.store_offsets) and not any(sb.store_offsets) and sa.grid != sb.grid and
+                  (sb.grid, 0) not in a.refs and (sa.grid, 0) not in b.refs)
+            if ok:
+                a.pair = f"xg_f{sa.sid}"
+                i += 2
+            else:
+                i += 1
 
     def _groups(self, stmt_meta: list) -> list:
         """Runs of consecutive stencil statements (same block, nothing between)."""
