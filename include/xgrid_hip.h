@@ -54,6 +54,8 @@ typedef struct xg_stmt_desc {
   int32_t overstep;                          /* 0 none, 1 limit, 2 wrap (xgrid/util/init.py:15) */
   int32_t group;                             /* fusion group (-1: none) */
   int32_t group_pos;
+  int32_t pair_fwd;                          /* fn_pair forwards this store to the next statement */
+  int32_t pair_pad;
   void* fn_main;                             /* hipFunction_t of the class kernel */
   void* fn_pingpong;                         /* hipFunction_t of the Jacobi sweep (PINGPONG) */
   void* fn_phase2;                           /* hipFunction_t storing a temporary (two-phase) */

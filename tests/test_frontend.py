@@ -301,20 +301,24 @@ def test_staged_lane_shifts_run_on_every_lane(double):
 
 
 def test_statement_pairs_marked(double):
-    """Consecutive in-place statements of one group storing to different grids,
-    neither loading the other's store grid at time 0, get a pair kernel."""
+    """Consecutive in-place statements of one group storing to different grids
+    get a pair kernel when the first does not load the second's store grid at
+    time 0 and the second loads the first's at time 0 only at its own cell
+    (then forwarded)."""
     from xgrid_amd import workloads as W
     from xgrid_amd.backend import lower_kernel
     low = lower_kernel(W.make_cavity())[0]
-    pairs = {m.stmt.sid: m.pair for m in low.stmts if m.pair}
-    assert pairs.get(11) == "xg_f11"                       # u and v updates
-    assert 0 not in pairs                                  # the first sweep reads b at time 0
-    assert "run_pair<xg_s11, xg_s12, XG_OVERSTEP>" in low.device_src
+    pairs = {m.stmt.sid: (m.pair, m.pair_fwd) for m in low.stmts if m.pair}
+    assert pairs.get(11) == ("xg_f11", False)              # u and v updates
+    assert pairs.get(0) == ("xg_f0", True)                 # b, then the first sweep reading b at its cell
+    assert low.stmts[1].fwd_from == "b"
+    assert "run_pair<xg_s11, xg_s12, XG_OVERSTEP, false>" in low.device_src
+    assert "run_pair<xg_s0, xg_s1, XG_OVERSTEP, true>" in low.device_src
     f2 = xgrid.grid[float, 2]
 
     @xgrid.kernel()
     def chain(u: f2, v: f2) -> None:
         u[0, 0] = v[0, 1]
-        v[0, 0] = u[0, 0][0] * 2.0                         # reads u at time 0: no pair
+        v[0, 0] = u[0, 1][0] * 2.0                         # reads u at time 0 at another cell: no pair
 
     assert not any(m.pair for m in lower_kernel(chain)[0].stmts)

@@ -291,14 +291,26 @@ def _pair_kernel():
     return two
 
 
-@pytest.mark.parametrize("case", ["paired", "aliased", "different_boxes"])
+def _forward_kernel():
+    @xgrid.kernel()
+    def fwd(u: f2, v: f2, w: f2) -> None:
+        u[0, 0] = w[0, 1] * 0.5 - w[-1, 0]
+        v[0, 0] = v[0, 1] + u[0, 0][0] * 3.0 + w[0, 0][0]    # u at time 0 at its own cell: forwarded
+        with xgrid.boundary(1):
+            u[0, 0] = 0.25
+    return fwd
+
+
+@pytest.mark.parametrize("case", ["paired", "aliased", "different_boxes", "forwarded", "forward_aliased"])
 def test_statement_pairs(double, case):
     """Two consecutive in-place statements over the same box run as one launch
-    (the second reads the first's store grid only at time -1).  When the same
-    grid is passed for both stores, or their classes are different boxes, they
-    run one after the other.  Every case equals the oracle bit for bit."""
+    (the second reads the first's store grid only at time -1, or at time 0 at
+    its own cell, where the first's value is forwarded).  When the same grid is
+    passed for both stores, another argument aliases the first's store, or the
+    classes are different boxes, they run one after the other.  Every case
+    equals the oracle bit for bit."""
     from xgrid_amd import workloads as W
-    k = _pair_kernel()
+    k = _forward_kernel() if case.startswith("forward") else _pair_kernel()
     rng = np.random.default_rng(21)
     n = (90, 130)                                # interiors above the chaining cap of 4096 cells
     walls = W.box_walls(n)
@@ -310,6 +322,8 @@ def test_statement_pairs(double, case):
     dw, hw = grid_pair(xgrid, n, float, None, rng.random(n))
     if case == "aliased":
         dargs, hargs = (du, du, dw), (hu, hu, hw)
+    elif case == "forward_aliased":
+        dargs, hargs = (du, dv, du), (hu, hv, hu)        # w[0, 0][0] then reads u's new values
     else:
         dargs, hargs = (du, dv, dw), (hu, hv, hw)
     run_pair(k, dargs, hargs, steps=5)
@@ -317,7 +331,7 @@ def test_statement_pairs(double, case):
         assert_bitexact(d, h, levels=[0])
     # a pair counts 2 fused statements per run (graph capture attempts run a call twice)
     fused = k.native.counters()["fused"]
-    assert (fused > 0) == (case == "paired"), fused
+    assert (fused > 0) == (case in ("paired", "forwarded")), fused
 
 
 def test_float_precision_mixed_literals(single):

@@ -117,6 +117,7 @@ class HipKernel:
             d.fn_pingpong_b = fn(m.kernel_pp + "b") if m.kernel_pp else None
             d.fn_phase2_b = fn(m.phase2 + "_b")
             d.fn_pair = fn(m.pair) if m.pair else None
+            d.pair_fwd = int(m.pair_fwd)
         groups = (_lib.GroupDesc * max(1, len(low.groups)))()
         for i, g in enumerate(low.groups):
             groups[i].first = g.first

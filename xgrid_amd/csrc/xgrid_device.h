@@ -368,10 +368,12 @@ __device__ void run_main(const xg_launch_hdr& H, const typename S::Refs& R, cons
 }
 
 /* Two in-place statements over the same box in one tile walk (the runtime
- * pairs them only when their classes are that same box and neither reads the
- * other's store grid at time 0, so the order of their stores does not matter).
- * Each work-item evaluates both statements at its cells, then stores both. */
-template <class SA, class SB, int OV>
+ * pairs them only when their classes are that same box and the first does not
+ * read the second's store grid at time 0).  Each work-item evaluates both
+ * statements at its cells, then stores both.  FWD: the second reads the first's
+ * store at time 0, at its own cell only; it takes the value the first just
+ * computed for that cell (SB::value_fwd) instead of loading it. */
+template <class SA, class SB, int OV, bool FWD>
 __device__ void run_pair(const xg_launch_hdr& HA, const typename SA::Refs& RA, const typename SA::Params& PA,
                          const xg_launch_hdr& HB, const typename SB::Refs& RB, const typename SB::Params& PB,
                          uint32_t tile) {
@@ -388,13 +390,15 @@ __device__ void run_pair(const xg_launch_hdr& HA, const typename SA::Refs& RA, c
 #pragma unroll
       for (int n = 0; n < NC; ++n) {
         va[n] = SA::template value<true>(RA, PA, HA.out.shape, tc.c[n][0], tc.c[n][1], tc.c[n][2]);
-        vb[n] = SB::template value<true>(RB, PB, HB.out.shape, tc.c[n][0], tc.c[n][1], tc.c[n][2]);
+        if constexpr (FWD) vb[n] = SB::template value_fwd<true>(RB, PB, HB.out.shape, tc.c[n][0], tc.c[n][1], tc.c[n][2], va[n]);
+        else vb[n] = SB::template value<true>(RB, PB, HB.out.shape, tc.c[n][0], tc.c[n][1], tc.c[n][2]);
       }
     } else {
 #pragma unroll
       for (int n = 0; n < NC; ++n) {
         va[n] = SA::template value<false>(RA, PA, HA.out.shape, tc.c[n][0], tc.c[n][1], tc.c[n][2]);
-        vb[n] = SB::template value<false>(RB, PB, HB.out.shape, tc.c[n][0], tc.c[n][1], tc.c[n][2]);
+        if constexpr (FWD) vb[n] = SB::template value_fwd<false>(RB, PB, HB.out.shape, tc.c[n][0], tc.c[n][1], tc.c[n][2], va[n]);
+        else vb[n] = SB::template value<false>(RB, PB, HB.out.shape, tc.c[n][0], tc.c[n][1], tc.c[n][2]);
       }
     }
 #pragma unroll

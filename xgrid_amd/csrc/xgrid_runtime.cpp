@@ -314,7 +314,10 @@ int exec_group(xg_kernel* k, int32_t gi, const std::vector<int>& sids, const std
 
 // Statements a and a + 1 run as one launch (the emitter's pair kernel) when both
 // run in place over the same box and no argument aliases a store grid the other
-// statement reads at time 0 or stores to.  A statement being timed runs alone.
+// statement reads at time 0 or stores to.  In a forwarding pair the second
+// statement's time-0 reads of the first's store argument (at its own cell) take
+// the computed value; any other argument bound to that grid still disqualifies.
+// A statement being timed runs alone.
 bool pairable(xg_kernel* k, int32_t a, int32_t b) {
   if (b != a + 1 || b >= (int32_t)k->stmts.size()) return false;
   StmtPlan& A = k->stmts[(size_t)a];
@@ -327,7 +330,9 @@ bool pairable(xg_kernel* k, int32_t a, int32_t b) {
   for (int r = 0; r < A.d.nrefs; ++r)
     if (A.d.ref_level[r] == 0 && k->grids[(size_t)A.d.ref_arg[r]] == GB) return false;
   for (int r = 0; r < B.d.nrefs; ++r)
-    if (B.d.ref_level[r] == 0 && k->grids[(size_t)B.d.ref_arg[r]] == GA) return false;
+    if (B.d.ref_level[r] == 0 && k->grids[(size_t)B.d.ref_arg[r]] == GA &&
+        !(A.d.pair_fwd && B.d.ref_arg[r] == A.d.store_arg))
+      return false;
   const ClassPlan& pa = GA->plan(A.d.klass);
   const ClassPlan& pb = GB->plan(B.d.klass);
   if (pa.kind != XG_DOM_BOX || pb.kind != XG_DOM_BOX || pa.count == 0) return false;

@@ -83,6 +83,8 @@ class StmtMeta:
     group: int = -1
     group_pos: int = 0
     pair: str | None = None       # kernel running this statement and the next in one launch
+    pair_fwd: bool = False        # ... with the next one's loads of this store (own cell, time 0) forwarded
+    fwd_from: str | None = None   # (second of a forwarding pair) the grid whose time-0 loads are forwarded
 
 
 @dataclass
@@ -457,7 +459,8 @@ class Emitter:
                 plan.add(refs.index((n.grid.name, n.level)), tuple(n.offsets))
         return plan if plan.usable() else None
 
-    def _device_stmt(self, st: StencilStmt, refs: list, params: list, out: list, tb: LoopMeta | None = None):
+    def _device_stmt(self, st: StencilStmt, refs: list, params: list, out: list, tb: LoopMeta | None = None,
+                     fwd_from: str | None = None):
         sid = st.sid
         D = st.dimension
         T = device_type_name(st.gtype.element)
@@ -468,8 +471,11 @@ class Emitter:
         pre_slots: list = []  # (grid, level, offsets) of each prefetched load
         rd_slots: list = []   # distinct (row, column) offsets of the loop grid's level-0 reads
         st3 = [None]          # while lowering value_st3: the staged 3-D plan (see _plan_st3)
+        fwd = [False]         # while lowering value_fwd: time-0 loads of `fwd_from` at the cell are `fwd`
 
         def cell(e: ir.Cell) -> str:
+            if fwd[0] and e.grid.name == fwd_from and e.level == 0 and not any(e.offsets):
+                return "fwd"
             if st3[0] is not None:
                 acc = st3[0].access(refs.index((e.grid.name, e.level)), tuple(e.offsets))
                 if acc is not None:
@@ -561,6 +567,20 @@ class Emitter:
         else:
             out.append("  static constexpr bool ST3 = false;")
             out.append("  static constexpr int ST3_LDS = 0;")
+        if fwd_from is not None:
+            # the same expression with its time-0 loads of `fwd_from` at the cell taken from the
+            # value the paired statement just computed for that cell
+            fwd[0] = True
+            expr_fwd = _ExprC(True, cell, name, grid_info)(st.node.value)
+            fwd[0] = False
+            ft = device_type_name(grids[fwd_from].element)
+            out.append("  template <bool SAFE>")
+            out.append(f"  __device__ static __forceinline__ T value_fwd(const Refs& R, const Params& P, const int32_t* sh, "
+                       f"int i0, int i1, int i2, const {ft} fwd) {{")
+            out.append(f"    const int64_t xc = SAFE ? xg::lin<{D}>(sh, i0, i1, i2) : 0;")
+            out.append("    (void)R; (void)P; (void)sh; (void)xc; (void)i0; (void)i1; (void)i2; (void)fwd;")
+            out.append(f"    return {expr_fwd};")
+            out.append("  }")
         if expr_tb is not None:
             # the same expression with the loop grid's level-0 reads taken from s[], the values
             # at the statement's distinct offsets (gathered from registers by the blocked kernel)
@@ -1079,14 +1099,15 @@ class Emitter:
         for op in self.device_functions:
             self._device_function(op, out)
         for m in stmt_meta:
-            self._device_stmt(m.stmt, m.refs, m.params, out, loop_of.get(m.stmt.sid))
+            self._device_stmt(m.stmt, m.refs, m.params, out, loop_of.get(m.stmt.sid), m.fwd_from)
         for m, nxt in zip(stmt_meta, stmt_meta[1:]):
             if m.pair:
                 a, b = m.stmt.sid, nxt.stmt.sid
                 out.append(f'extern "C" __global__ void __launch_bounds__(256) {m.pair}('
                            f"const xg_launch_hdr HA, const xg_r{a} RA, const xg_p{a} PA, "
                            f"const xg_launch_hdr HB, const xg_r{b} RB, const xg_p{b} PB) "
-                           f"{{ xg::run_pair<xg_s{a}, xg_s{b}, XG_OVERSTEP>(HA, RA, PA, HB, RB, PB, "
+                           f"{{ xg::run_pair<xg_s{a}, xg_s{b}, XG_OVERSTEP, {'true' if m.pair_fwd else 'false'}>"
+                           f"(HA, RA, PA, HB, RB, PB, "
                            f"xg::xcd_remap(blockIdx.x, gridDim.x)); }}")
         for L in loops:
             for V in (L, L.alt):
@@ -1449,10 +1470,12 @@ class Emitter:
 
     def _pairs(self, stmt_meta: list) -> None:
         """Mark statement pairs that may run as one launch: consecutive in-place 1-D or
-        2-D statements of one group, storing unshifted to different grids, neither
-        loading the other's store grid at time 0 (the cavity's u and v updates).  The
-        runtime runs a pair together only when both classes are the same box and no
-        two arguments alias (`pairable` in xgrid_runtime.cpp)."""
+        2-D statements of one group, storing unshifted to different grids, the first
+        not loading the second's store grid at time 0 (the cavity's u and v updates).
+        The second may load the first's store at time 0 at its own cell only: that
+        value is forwarded in a register (the cavity's b and first pressure sweep).
+        The runtime runs a pair together only when both classes are the same box and
+        no two arguments alias (`pairable` in xgrid_runtime.cpp)."""
         i = 0
         while i + 1 < len(stmt_meta):
             a, b = stmt_meta[i], stmt_meta[i + 1]
@@ -1461,9 +1484,16 @@ class Emitter:
                   sa.mode == "inplace" and sb.mode == "inplace" and
                   sa.dimension == sb.dimension and sa.dimension in (1, 2) and
                   not any(sa.store_offsets) and not any(sb.store_offsets) and sa.grid != sb.grid and
-                  (sb.grid, 0) not in a.refs and (sa.grid, 0) not in b.refs)
+                  (sb.grid, 0) not in a.refs)
+            # b may read a's store at time 0 only at its own cell: that value is forwarded
+            fwd = ok and (sa.grid, 0) in b.refs
+            if fwd:
+                ok = all(not any(n.offsets) for n in ir.walk(sb.node.value)
+                         if isinstance(n, ir.Cell) and n.grid.name == sa.grid and n.level == 0)
             if ok:
                 a.pair = f"xg_f{sa.sid}"
+                a.pair_fwd = fwd
+                b.fwd_from = sa.grid if fwd else None
                 i += 2
             else:
                 i += 1
