@@ -35,7 +35,7 @@ class Configuration:
     device: int = 0
     fp_contract: bool = False      # FMA contraction in device code
     graphs: bool = True            # replay repeated calls as hipGraphs
-    fuse: bool = True              # chain small boundary statements in one workgroup
+    fuse: bool = True              # chain small boundary statements in one workgroup; run statement pairs
     blocked: bool = True           # run blockable stencil loops temporally blocked
 
     def __repr__(self) -> str:
