@@ -148,6 +148,24 @@ def test_overstep_modes(mode):
         assert abs(d.now.sum() - h.now.sum()) < 1e-9
 
 
+@pytest.mark.parametrize("mode", ["wrap", "limit"])
+def test_overstep_modes_3d_staged(mode):
+    """FP64 3-D statements with every cell in class 0: tiles away from the faces
+    take the staged loads, tiles at the faces the overstep rule (wrap or clamp)."""
+    xgrid.init(precision="double", overstep=mode)
+
+    @xgrid.kernel()
+    def smooth3(u: f3, c: float) -> None:
+        u[0, 0, 0] = u[0, 0, 0] + c * (u[1, 0, 0] + u[-1, 0, 0] + u[0, 1, 0] + u[0, -1, 0] +
+                                       u[0, 0, 1] + u[0, 0, -1] - 6.0 * u[0, 0, 0])
+
+    rng = np.random.default_rng(6)
+    shape = (12, 13, 150)
+    d, h = grid_pair(xgrid, shape, float, None, rng.random(shape))
+    run_pair(smooth3, (d, 0.1), (h, 0.1), steps=4)
+    assert_bitexact(d, h)
+
+
 def test_twophase_store_offset(double):
     @xgrid.kernel()
     def shift(u: f1) -> None:
