@@ -39,9 +39,10 @@ def test_struct_layouts_match_c():
 #include "xgrid_hip.h"
 #include "xgrid_abi.h"
 int main(void) {
-  printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(xg_stmt_desc), offsetof(xg_stmt_desc, param_bytes),
+  printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(xg_stmt_desc), offsetof(xg_stmt_desc, param_bytes),
          offsetof(xg_stmt_desc, fn_main), sizeof(xg_group_desc), sizeof(xg_api), sizeof(xg_dom),
-         sizeof(xg_gview), sizeof(xg_launch_hdr), offsetof(xg_dom, count));
+         sizeof(xg_gview), sizeof(xg_launch_hdr), offsetof(xg_dom, count), offsetof(xg_stmt_desc, pair_fwd),
+         offsetof(xg_stmt_desc, fn_pair), sizeof(xg_loop_desc), offsetof(xg_loop_desc, alt_max_cells));
   return 0;
 }
 '''
@@ -53,7 +54,9 @@ int main(void) {
                         "-o", exe], check=True)
         got = [int(x) for x in subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split()]
     exp = [ctypes.sizeof(_lib.StmtDesc), _lib.StmtDesc.param_bytes.offset, _lib.StmtDesc.fn_main.offset,
-           ctypes.sizeof(_lib.GroupDesc), ctypes.sizeof(_lib.Api), 72, 24, 120, 32]
+           ctypes.sizeof(_lib.GroupDesc), ctypes.sizeof(_lib.Api), 72, 24, 120, 32,
+           _lib.StmtDesc.pair_fwd.offset, _lib.StmtDesc.fn_pair.offset,
+           ctypes.sizeof(_lib.LoopDesc), _lib.LoopDesc.alt_max_cells.offset]
     assert got == exp
 
 
