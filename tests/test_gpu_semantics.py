@@ -166,6 +166,30 @@ def test_overstep_modes_3d_staged(mode):
     assert_bitexact(d, h)
 
 
+def test_3d_staged_neighbours_of_other_planes(double):
+    """Staged FP64 loads of row and column neighbours on the planes above and
+    below (LDS copies and lane shifts of planes other than the cell's own),
+    with a diagonal load left global, equal the oracle."""
+    @xgrid.kernel()
+    def k(u: f3, v: f3) -> None:
+        u[0, 0, 0] = (v[1, 1, 0] + v[-1, 0, 1] * 0.5 + v[1, -1, 0] * 0.25 - v[-1, 0, -1] + v[0, 0, 0] +
+                      v[0, 1, 0] * 0.125 - v[1, 1, 1])
+        with xgrid.boundary(1):
+            u[0, 0, 0] = v[0, 0, 0] * 2.0
+
+    from xgrid_amd import workloads as W
+    from xgrid_amd.backend import lower_kernel
+    src = lower_kernel(k)[0].device_src
+    assert "ST3 = true" in src and "ST3_LDS = 24576" in src       # two plane offsets staged in LDS
+    rng = np.random.default_rng(12)
+    shape = (14, 19, 140)
+    du, hu = grid_pair(xgrid, shape, float, W.box_walls(shape), rng.random(shape))
+    dv, hv = grid_pair(xgrid, shape, float, None, rng.random(shape))
+    run_pair(k, (du, dv), (hu, hv), steps=3)
+    assert_bitexact(du, hu)
+    assert_bitexact(dv, hv)
+
+
 def test_twophase_store_offset(double):
     @xgrid.kernel()
     def shift(u: f1) -> None:
