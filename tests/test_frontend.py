@@ -194,7 +194,7 @@ def test_fill_default_time_truncates_and_raises(double):
 
 
 def _stmt_struct(src: str, sid: int) -> str:
-    i = src.index(f"struct xg_s{sid} ")
+    i = src.index(f"struct xg_s{sid}_t ")
     return src[i:src.index("extern", i)]
 
 

@@ -50,6 +50,56 @@ def test_cavity_full_size_few_steps(double):
         assert_bitexact(*pairs[c], levels=[0])
 
 
+def _cavity_live(xg, n, steps, seed):
+    """The cavity on the device and on the oracle from seeded non-zero interior
+    velocities, so that `b` and `p` are non-zero from the first call and every
+    blocked pressure launch works on live data."""
+    from xgrid_amd import workloads as W
+    k = W.make_cavity()
+    cfg = W.cavity_config(n, dt=2e-6)
+    masks = W.cavity_masks(n)
+    rng = np.random.default_rng(seed)
+    init = {"b": None, "p": None,
+            "u": rng.uniform(-0.5, 0.5, (n, n)), "v": rng.uniform(-0.5, 0.5, (n, n))}
+    pairs = {c: grid_pair(xg, (n, n), float, masks[c], init[c]) for c in "bpuv"}
+    run_pair(k, [pairs[c][0] for c in "bpuv"] + [cfg], [pairs[c][1] for c in "bpuv"] + [cfg], steps=steps)
+    return k, pairs
+
+
+@pytest.mark.parametrize("n", [2049, 1501])
+def test_cavity_headline_tile_on_live_data(double, n):
+    """The 2049^2 FP64 cavity the benchmark times, on the blocked-loop tile it
+    times (48 x 40 core, 1024 work-items, 10 iterations per launch; plan 0), from
+    non-zero velocities: b and p are live in every blocked launch.  2049^2 has
+    power-of-two spacings (dx = dy = 2^-10), so its uniform divisions take the
+    reciprocal form; 1501^2 (dx = 2/1500) keeps the divisions."""
+    k, pairs = _cavity_live(double, n, 3, seed=7)
+    L = k.native.lowered.loops[0]
+    assert (L.core, L.threads, L.tmax, L.variant) == ((48, 40), 1024, 10, "lds")
+    assert k.native.loop_plan(0, n * n) == 0
+    info = k.native.loop_info(0)
+    assert info["state"] == 1 and info["iterations"] == 50 * 3
+    assert np.abs(pairs["p"][1].now).max() > 0 and np.abs(pairs["b"][1].now).max() > 0
+    for c in "bpuv":
+        assert_bitexact(*pairs[c], levels=[0])
+
+
+def test_cavity_fp32_headline_tile_on_live_data(single):
+    """The 2049^2 cavity with FP32 fields on its blocked tile (72 x 40 core, 1024
+    work-items; plan 0, not the small-grid tile) from non-zero velocities, bit for
+    bit against the oracle in FP32."""
+    n = 2049
+    k, pairs = _cavity_live(single, n, 3, seed=8)
+    L = k.native.lowered.loops[0]
+    assert (L.core, L.threads, L.tmax, L.variant) == ((72, 40), 1024, 10, "lds")
+    assert k.native.loop_plan(0, n * n) == 0
+    info = k.native.loop_info(0)
+    assert info["state"] == 1 and info["iterations"] == 50 * 3
+    assert pairs["p"][0].now.dtype == np.float32
+    for c in "bpuv":
+        assert_bitexact(*pairs[c], levels=[0])
+
+
 @pytest.mark.parametrize("n", [34, 130])
 @pytest.mark.parametrize("prec", ["double", "float"])
 def test_heat_3d(prec, n, request):

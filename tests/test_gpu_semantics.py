@@ -466,7 +466,10 @@ def test_division_by_scalar_is_ieee_for_every_double(double):
     a = xgrid.Grid((n,), float)
     a.now[:] = vals
     u = xgrid.Grid((n,), float)
-    for c in (3.0, 0.1, -7.25, 1e-5, 6.02e23, 2.0 ** -120, 2.0 ** 150, -1e300, 0.0):
+    # powers of two take the reciprocal form (the largest, 2^1023, has the subnormal
+    # reciprocal 2^-1023); the subnormal 2^-1023 itself and the other divisors divide
+    for c in (3.0, 0.1, -7.25, 1e-5, 6.02e23, 2.0 ** -120, 2.0 ** 150, -1e300, 0.0,
+              1.0, -0.5, 2.0 ** -10, 2.0 ** -18, 2.0 ** -1022, -(2.0 ** 1023), 2.0 ** 1000, 2.0 ** -1023, np.inf):
         divk(u, a, c)
         with np.errstate(all="ignore"):
             want = vals / c
@@ -474,6 +477,72 @@ def test_division_by_scalar_is_ieee_for_every_double(double):
         nan = np.isnan(want)
         assert np.array_equal(np.isnan(got), nan), c
         assert np.array_equal(got[~nan].view(np.uint64), want[~nan].view(np.uint64)), c
+
+
+def test_division_by_scalar_is_ieee_for_every_float(single):
+    """The FP32 counterpart: float grids divided by a float scalar, including the
+    power-of-two divisors of the reciprocal form (2^127: subnormal reciprocal
+    2^-127) and subnormal results, against numpy's IEEE float32 division."""
+    @xgrid.kernel()
+    def divf(u: f1, a: f1, c: float) -> None:
+        u[0] = a[0][0] / c
+
+    rng = np.random.default_rng(22)
+    n = 1 << 20
+    vals = rng.integers(0, 2**31, n, dtype=np.int64).astype(np.uint32).view(np.float32).copy()
+    vals[: n // 2] = (rng.random(n // 2) * 10.0 ** rng.integers(-20, 20, n // 2)).astype(np.float32)
+    vals[::3] = np.negative(vals[::3])
+    vals[:8] = np.array([0.0, -0.0, np.inf, -np.inf, np.nan, 1e-45, 3.4028235e38, 2.0 ** -100], np.float32)
+    a = xgrid.Grid((n,), float)
+    a.now[:] = vals
+    u = xgrid.Grid((n,), float)
+    assert a.now.dtype == np.float32
+    for c in (3.0, 0.1, -7.25, 2.0 ** -3, 2.0 ** 20, -(2.0 ** 127), 2.0 ** -126, 2.0 ** -127, 1.0, 0.0):
+        cf = np.float32(c)
+        divf(u, a, float(cf))
+        with np.errstate(all="ignore"):
+            want = vals / cf
+        got = u.now
+        assert got.dtype == np.float32
+        nan = np.isnan(want)
+        assert np.array_equal(np.isnan(got), nan), c
+        assert np.array_equal(got[~nan].view(np.uint32), want[~nan].view(np.uint32)), c
+
+
+def test_blocked_loop_revalidates_longer_launches(double):
+    """A blocked plan proven on a 3-iteration launch is not trusted for 10: the
+    boundary row below moves data 3 cells per iteration (classes 1, 2, 3 each
+    read the cell the previous statement just wrote), which a 12-cell halo holds
+    for 3 iterations but not for 10.  The first call (trip 3) validates and runs
+    blocked; the second (trip 12) must check again, decline, and run unblocked,
+    with the oracle's bits in both."""
+    @xgrid.kernel()
+    def creep_n(u: f2, v: f2, trip: int) -> None:
+        for _ in range(0, trip):
+            u[0, 0] = (u[0, 1][0] + u[0, -1][0] + u[1, 0][0] + u[-1, 0][0]) * 0.25 + v[0, 0]
+            with xgrid.boundary(1):
+                u[0, 0] = u[0, -1][0] + 1.0
+            with xgrid.boundary(2):
+                u[0, 0] = u[0, -1][0] * 0.5
+            with xgrid.boundary(3):
+                u[0, 0] = u[0, -1][0] - 0.25
+
+    n = 150
+    mask = np.zeros((n, n), np.int32)
+    mask[0, :] = 1 + np.arange(n) % 3
+    mask[-1, :] = mask[:, 0] = mask[:, -1] = 4
+    rng = np.random.default_rng(13)
+    du, hu = grid_pair(xgrid, (n, n), float, mask, rng.random((n, n)))
+    dv, hv = grid_pair(xgrid, (n, n), float, None, rng.random((n, n)) * 1e-3)
+    run_pair(creep_n, (du, dv, 3), (hu, hv, 3), steps=1)
+    assert creep_n.native.lowered.loops[0].tmax == 10
+    info = creep_n.native.loop_info(creep_n.native.loop_plan(0, n * n))
+    assert info["state"] == 1 and info["iterations"] == 3
+    assert_bitexact(du, hu)
+    run_pair(creep_n, (du, dv, 12), (hu, hv, 12), steps=1)
+    info = creep_n.native.loop_info(creep_n.native.loop_plan(0, n * n))
+    assert info["state"] == -1 and info["iterations"] == 3
+    assert_bitexact(du, hu)
 
 
 def test_pointer_argument(double):
@@ -563,7 +632,7 @@ def test_blocked_loop_declined_when_halo_too_small(double):
 
 @pytest.mark.parametrize("shape", [(40, 64, 512, 6, "reg"), (24, 64, 512, 4, "reg"), (48, 64, 768, 10, "reg"),
                                    (32, 64, 512, 8, "lds"), (24, 64, 256, 6, "lds"), (72, 64, 1024, 10, "lds"),
-                                   (48, 64, 768, 10, "lds")])
+                                   (48, 64, 768, 10, "lds"), (48, 64, 1024, 10, "lds")])
 def test_blocked_loop_shapes_and_layouts_match_oracle(double, shape):
     """Every tile shape and both layouts (region in LDS / columns in registers)
     of the blocked loop give the oracle's bits, on grids whose edges cut tiles."""

@@ -111,6 +111,7 @@ def tune_batch(op, make_instance, instances: list, precision: str, candidates: l
         native = HipKernel(op, shape)
         if not native.lowered.loops:
             timings[",".join(map(str, shape))] = None
+            native.close()
             continue
         b = Batch(op, native=native, max_blocks=max_blocks)
         for it in instances:
@@ -123,6 +124,7 @@ def tune_batch(op, make_instance, instances: list, precision: str, candidates: l
         _lib.check(lib.xg_synchronize())
         timings[",".join(map(str, shape))] = (time.perf_counter() - t0) / steps * 1e3
         b.close()
+        native.close()            # its instance plans went with the batch
     valid = {k: v for k, v in timings.items() if v is not None}
     if not valid:
         return {"shape": None, "timings": timings, "cached": False}

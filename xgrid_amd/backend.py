@@ -11,6 +11,7 @@ fallback: without the library or a GPU this raises.
 from __future__ import annotations
 
 import ctypes
+import weakref
 from ctypes import c_void_p
 
 from . import _lib
@@ -50,8 +51,25 @@ def packed_args_type(arg_types):
     return type("xg_args", (ctypes.Structure,), {"_fields_": fields_})
 
 
+def _release(lib, plan, module):
+    """Destroy a kernel's own launch plan (its graph execs, events, flag) and unload
+    its code object.  Plans made by new_plan() for batch instances belong to their
+    batch, which holds the kernel and destroys them first."""
+    try:
+        if plan:
+            lib.xg_kernel_destroy(plan)
+        if module:
+            lib.xg_module_unload(module)
+    except Exception:
+        pass
+
+
 class HipKernel:
-    """A compiled kernel: code object, stub entry point and launch plan."""
+    """A compiled kernel: code object, stub entry point and launch plan.
+
+    close() (or the garbage collector, through a finalizer) destroys the plan and
+    unloads the code object.  The finalizer does not run at interpreter exit: the
+    process's end releases both, with no GPU calls during shutdown."""
 
     def __init__(self, op, block_shape: tuple | None = None):
         cfg = get_config()
@@ -68,6 +86,9 @@ class HipKernel:
         mod = c_void_p()
         _lib.check(self.lib.xg_module_load(self.code_object.encode(), ctypes.byref(mod)), "xg_module_load")
         self.module = mod.value
+        self.kernel = None
+        self._finalizer = weakref.finalize(self, _release, self.lib, None, self.module)
+        self._finalizer.atexit = False
 
         def fn(name):
             if name is None:
@@ -151,6 +172,9 @@ class HipKernel:
         self._nloops = len(low.loops) + len(alts)
         self._descs, self._groups, self._loops = descs, groups, loops
         self.kernel = self.new_plan()
+        self._finalizer.detach()
+        self._finalizer = weakref.finalize(self, _release, self.lib, self.kernel, self.module)
+        self._finalizer.atexit = False
         self.set_options()
         self.api = self.lib.xg_api_table()
         self.stub = load_host(self.stub_path)
@@ -166,6 +190,13 @@ class HipKernel:
         self.Args = packed_args_type(self.arg_types)
         if ctypes.sizeof(self.Args) != self.stub.xg_args_size():
             raise KernelError("packed argument layout differs between the stub and ctypes")
+
+    def close(self) -> None:
+        """Destroy the launch plan and unload the code object now.  Batch plans made
+        with new_plan() must have been destroyed already (Batch.close)."""
+        self._finalizer()
+        self.kernel = None
+        self.module = None
 
     def new_plan(self) -> int:
         """A fresh launch plan (xg_kernel) over this kernel's code: one per batch instance."""

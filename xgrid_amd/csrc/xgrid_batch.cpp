@@ -373,10 +373,13 @@ int run_loop(xg_batch* b, int32_t lid, int64_t trip, const std::vector<int>& idx
   for (size_t t = 0; t < idx.size(); ++t) {
     xg_kernel* k = b->inst[(size_t)idx[t]].k;
     LoopPlan& L = k->loops[(size_t)lid];
-    if (!k->blocked || trip <= 0 || !L.d.fn_b || !L.d.fn_check_b || !loop_blockable(k, L) || L.state < 0 ||
-        (L.state == 0 && b->mode == kCapture))
+    const bool can = k->blocked && trip > 0 && L.d.fn_b && L.d.fn_check_b && loop_blockable(k, L) && L.state >= 0;
+    // a launch of more iterations than the plan's checks ran needs a check of its own
+    // (a check needs a synchronisation, so none is captured)
+    const bool unproven = can && (L.state == 0 || std::min<int64_t>(L.d.tmax, trip) > L.validated);
+    if (!can || (unproven && b->mode == kCapture))
       plain.push_back((int)t);
-    else if (L.state == 0)
+    else if (unproven)
       check.push_back((int)t);
     else
       blocked.push_back((int)t);
@@ -465,6 +468,7 @@ int run_loop(xg_batch* b, int32_t lid, int64_t trip, const std::vector<int>& idx
         plain.push_back(check[c]);
       } else {
         L.state = 1;
+        L.validated = std::max<int>(L.validated, (int)std::min<int64_t>(L.d.tmax, trip));
         passed.push_back(check[c]);
       }
     }

@@ -528,6 +528,62 @@ __device__ void run_block(const xg_launch_hdr& H, const typename S::Refs& R, con
 template <class T>
 __device__ __forceinline__ T sq(T x) { return x * x; }
 
+/* ---- division by a launch-uniform divisor ---------------------------------
+ * A statement's per-cell divisions whose divisor holds no grid load have one
+ * divisor per launch.  When that divisor d is +-2^k with 2^-k representable
+ * (biased exponent 1..2046 in FP64, 1..254 in FP32; 2^-1023 and 2^-127 are
+ * exact subnormals), r = 1/d is exact and a * r is the same real number as
+ * a / d for every a, so IEEE multiplication and division round it to the same
+ * result, with the same infinities, signed zeros, subnormals and overflow.
+ * Kernels test the divisors once per workgroup (dv_ok) and run the
+ * instantiation DV = true only when all of them pass; the reciprocal is built
+ * from the exponent bits, with no division. */
+template <class T> struct is_fp { static constexpr bool value = false; };
+template <> struct is_fp<float> { static constexpr bool value = true; };
+template <> struct is_fp<double> { static constexpr bool value = true; };
+
+__device__ __forceinline__ bool rcp_exact(double d) {
+  uint64_t b;
+  __builtin_memcpy(&b, &d, 8);
+  const uint64_t e = (b >> 52) & 0x7ffull;
+  return (b & 0xfffffffffffffull) == 0 && e >= 1 && e <= 2046;
+}
+__device__ __forceinline__ bool rcp_exact(float d) {
+  uint32_t b;
+  __builtin_memcpy(&b, &d, 4);
+  const uint32_t e = (b >> 23) & 0xffu;
+  return (b & 0x7fffffu) == 0 && e >= 1 && e <= 254;
+}
+template <class I>
+__device__ __forceinline__ bool rcp_exact(I) { return false; }   /* integer divisors: no reciprocal form */
+
+/* 1/d for d passing rcp_exact. */
+__device__ __forceinline__ double rcp_pow2(double d) {
+  uint64_t b;
+  __builtin_memcpy(&b, &d, 8);
+  const uint64_t e = (b >> 52) & 0x7ffull;
+  const uint64_t r = (b & 0x8000000000000000ull) | (e < 2046 ? (2046 - e) << 52 : 1ull << 51);
+  double x;
+  __builtin_memcpy(&x, &r, 8);
+  return x;
+}
+__device__ __forceinline__ float rcp_pow2(float d) {
+  uint32_t b;
+  __builtin_memcpy(&b, &d, 4);
+  const uint32_t e = (b >> 23) & 0xffu;
+  const uint32_t r = (b & 0x80000000u) | (e < 254 ? (254 - e) << 23 : 1u << 22);
+  float x;
+  __builtin_memcpy(&x, &r, 4);
+  return x;
+}
+
+template <bool DV, class A, class B>
+__device__ __forceinline__ auto udiv(A a, B b) -> decltype(a / b) {
+  typedef decltype(a / b) R;
+  if constexpr (DV && is_fp<R>::value && is_fp<B>::value) return (R)a * rcp_pow2((R)b);
+  else return a / b;
+}
+
 /* A compile-time boolean carried as a value (selects a template instantiation). */
 template <bool B>
 struct bconst {

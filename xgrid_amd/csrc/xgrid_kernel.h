@@ -52,6 +52,7 @@ struct LoopPlan {
   const void* grid = nullptr;   // the grid the plan was validated on
   uint64_t mask_version = 0;    // ... and its mask
   int state = 0;                // 1 exact, 0 unknown, -1 declined
+  int validated = 0;            // state 1: launches of at most this many iterations are proven exact
   int64_t launches = 0, iterations = 0;
   std::vector<std::vector<uint8_t>> last_params;
 };

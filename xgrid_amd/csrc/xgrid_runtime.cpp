@@ -435,6 +435,7 @@ bool loop_blockable(xg_kernel* k, LoopPlan& L) {
   }
   if (L.grid != (const void*)G || L.mask_version != G->mask_version) {
     L.state = 0;
+    L.validated = 0;
     L.grid = G;
     L.mask_version = G->mask_version;
   }
@@ -729,14 +730,19 @@ int api_loop(xg_kernel* k, int32_t lid, int64_t trip, const void* const* params,
   xg_grid* G = k->grids[(size_t)L.d.grid_arg];
   if (!loop_blockable(k, L)) return 0;
   if (L.state < 0) return 0;
-  if (L.state == 0 && k->mode == kCapture) return 0;  // validation needs a synchronisation
+  // A check proves launches of up to the iterations it ran: the set of exact region
+  // cells only shrinks from one iteration to the next, so a proof at T covers fewer
+  // iterations, but a launch of more iterations than any check ran needs its own.
+  const int T0 = (int)std::min<int64_t>(L.d.tmax, trip);
+  const bool check = L.state == 0 || T0 > L.validated;
+  if (check && k->mode == kCapture) return 0;  // validation needs a synchronisation
   int rc = flush(k);
   if (rc) {
     set_status(k, rc);
     return 1;
   }
   int64_t done = 0;
-  if (L.state == 0) {
+  if (check) {
     if (!k->d_fail) {
       void* p = nullptr;
       if (hipMalloc(&p, sizeof(int32_t)) != hipSuccess) {
@@ -746,7 +752,7 @@ int api_loop(xg_kernel* k, int32_t lid, int64_t trip, const void* const* params,
       k->d_fail = (int32_t*)p;
     }
     int32_t h_fail = 0;
-    const int T = (int)std::min<int64_t>(L.d.tmax, trip);
+    const int T = T0;
     if (hipMemsetAsync(k->d_fail, 0, sizeof(int32_t), g_dev.stream) != hipSuccess ||
         (rc = launch_blocked(k, L, G, params, T, true)) != XG_OK ||
         hipMemcpyAsync(&h_fail, k->d_fail, sizeof(int32_t), hipMemcpyDeviceToHost, g_dev.stream) != hipSuccess ||
@@ -759,6 +765,7 @@ int api_loop(xg_kernel* k, int32_t lid, int64_t trip, const void* const* params,
       return 0;
     }
     L.state = 1;
+    L.validated = std::max(L.validated, T);
     std::swap(G->levels[0], G->scratch);
     done = T;
     L.iterations += T;
@@ -905,6 +912,7 @@ int xg_module_function(xg_module* m, const char* name, void** out_fn) {
 
 int xg_module_unload(xg_module* m) {
   if (!m) return XG_OK;
+  if (g_dev.ready) XG_HIP(hipStreamSynchronize(g_dev.stream));  // its kernels may still run
   XG_HIP(hipModuleUnload(m->mod));
   delete m;
   return XG_OK;

@@ -129,11 +129,14 @@ class Lowered:
 class _ExprC:
     """Expression text with C semantics, shared by host and device lowering."""
 
-    def __init__(self, device: bool, cell=None, name=None, grid_info=None):
+    def __init__(self, device: bool, cell=None, name=None, grid_info=None, udivs: list | None = None):
         self.device = device
         self.cell = cell              # callback for ir.Cell (device statements only)
         self.name = name              # callback for ir.Name
         self.grid_info = grid_info    # callback for ir.GridInfo
+        # device statements: divisor texts of the per-cell divisions by a launch-uniform
+        # value, which are emitted as xg::udiv<DV> (see _uniform_division)
+        self.udivs = udivs
 
     def __call__(self, e) -> str:
         m = getattr(self, "e_" + type(e).__name__, None)
@@ -170,6 +173,11 @@ class _ExprC:
             if fold == "inv":
                 return f"(({'1.0' if double else '1.0f'}) / {x})"
             return f"{'pow' if double else 'powf'}({x}, (({fp})({self(e.right)})))"
+        if self.udivs is not None and _uniform_division(e):
+            d = self(e.right)
+            if d not in self.udivs:
+                self.udivs.append(d)
+            return f"xg::udiv<DV>({self(e.left)}, {d})"
         return f"({self(e.left)} {e.op} {self(e.right)})"
 
     def e_Un(self, e):
@@ -194,6 +202,19 @@ class _ExprC:
         if e.info == "dimension":
             return str(e.grid.type.dimension)
         return self.grid_info(e)
+
+
+def _uniform_division(e) -> bool:
+    """A floating-point division whose dividend loads a grid (a value per cell) and
+    whose divisor loads none (one value per launch: parameters, constants, shapes).
+    Such a divisor is tested once per workgroup; when it is a power of two whose
+    reciprocal is a normal or exact subnormal number, the kernel multiplies by
+    that reciprocal instead, which is exactly IEEE division there (see
+    xg::udiv)."""
+    if e.op != "/" or not isinstance(e.type, Floating):
+        return False
+    return any(isinstance(n, ir.Cell) for n in ir.walk(e.left)) and \
+        not any(isinstance(n, ir.Cell) for n in ir.walk(e.right))
 
 
 def function_symbol(op) -> str:
@@ -392,6 +413,7 @@ class Emitter:
         self.device_functions = []     # those reachable from stencil expressions
         self._prefetched = set()       # blocked-loop statements whose other-grid loads are prefetched
         self._tb_reads = {}            # blocked-loop statement -> its loop-grid read offsets, in slot order
+        self._dv = {}                  # statement -> has a reciprocal form (uniform divisors; see udiv)
         self._collect_functions()
 
     # ------------------------------------------------------------- helpers
@@ -508,28 +530,29 @@ class Emitter:
             r = next(i for i, ref in enumerate(refs) if ref[0] == e.grid.name)
             return f"R.v[{r}].shape[{_ExprC(True, cell, name, grid_info)(e.dim)}]"
 
-        expr = _ExprC(True, cell, name, grid_info)(st.node.value)
+        udivs: list = []      # divisor texts of the per-cell divisions by a launch-uniform value
+        expr = _ExprC(True, cell, name, grid_info, udivs)(st.node.value)
         expr_st3 = None
         plan3 = self._plan_st3(st, refs, grids) if D == 3 and tb is None else None
         if plan3 is not None:
             st3[0] = plan3
-            expr_st3 = _ExprC(True, cell, name, grid_info)(st.node.value)
+            expr_st3 = _ExprC(True, cell, name, grid_info, udivs)(st.node.value)
             st3[0] = None
         expr_tb = expr_pre = expr_lds = expr_lds_pre = None
         if tb is not None:
             in_reg[0] = True
-            expr_tb = _ExprC(True, cell, name, grid_info)(st.node.value)
+            expr_tb = _ExprC(True, cell, name, grid_info, udivs)(st.node.value)
             in_reg[0] = False
             in_lds[0] = True
-            expr_lds = _ExprC(True, cell, name, grid_info)(st.node.value)
+            expr_lds = _ExprC(True, cell, name, grid_info, udivs)(st.node.value)
             in_lds[0] = False
             if st.klass == 0:
                 pre_mode[0] = True
                 in_reg[0] = True
-                expr_pre = _ExprC(True, cell, name, grid_info)(st.node.value)
+                expr_pre = _ExprC(True, cell, name, grid_info, udivs)(st.node.value)
                 in_reg[0] = False
                 in_lds[0] = True
-                expr_lds_pre = _ExprC(True, cell, name, grid_info)(st.node.value)
+                expr_lds_pre = _ExprC(True, cell, name, grid_info, udivs)(st.node.value)
                 in_lds[0] = False
                 pre_mode[0] = False
                 if len(pre_slots) > self.PREFETCH_MAX:
@@ -550,8 +573,15 @@ class Emitter:
         else:
             out.append(f"struct xg_p{sid} {{ int32_t xg_unused; }};")
         out.append(f"struct xg_r{sid} {{ xg_gview v[{nr}]; }};")
-        out.append(f"struct xg_s{sid} {{")
+        # DV: the statement's uniform divisors were found to be exact powers of two for
+        # this launch (dv_ok), so xg::udiv multiplies by their reciprocals
+        out.append(f"template <bool DV> struct xg_s{sid}_t {{")
         out.append(f"  typedef {T} T; typedef xg_r{sid} Refs; typedef xg_p{sid} Params;")
+        out.append(f"  static constexpr int NDV = {len(udivs)};")
+        out.append("  __device__ static __forceinline__ bool dv_ok(const Refs& R, const Params& P) {")
+        out.append("    (void)R; (void)P;")
+        out.append(f"    return {' && '.join(f'xg::rcp_exact({d})' for d in udivs) if udivs else 'false'};")
+        out.append("  }")
         out.append(f"  static constexpr int D = {D}, SO0 = {so[0]}, SO1 = {so[1]}, SO2 = {so[2]};")
         out.append(f"  static constexpr int RL0 = {rl[0]}, RL1 = {rl[1]}, RL2 = {rl[2]}, "
                    f"RH0 = {rh[0]}, RH1 = {rh[1]}, RH2 = {rh[2]};")
@@ -571,7 +601,7 @@ class Emitter:
             # the same expression with its time-0 loads of `fwd_from` at the cell taken from the
             # value the paired statement just computed for that cell
             fwd[0] = True
-            expr_fwd = _ExprC(True, cell, name, grid_info)(st.node.value)
+            expr_fwd = _ExprC(True, cell, name, grid_info, udivs)(st.node.value)
             fwd[0] = False
             ft = device_type_name(grids[fwd_from].element)
             out.append("  template <bool SAFE>")
@@ -632,6 +662,20 @@ class Emitter:
             out.append("  }")
             self._prefetched.add(sid)
         out.append("};")
+        out.append(f"typedef xg_s{sid}_t<false> xg_s{sid}; typedef xg_s{sid}_t<true> xg_s{sid}_r;")
+        # a staged 3-D statement keeps its LDS copy in run_main: a second instantiation
+        # would allocate a second copy, so it always divides
+        dv = bool(udivs) and expr_st3 is None
+        self._dv[sid] = dv
+
+        def disp(call: str, rv: str, pv: str) -> str:
+            """`call` (with {S} for the statement type), taking the reciprocal form when
+            this launch's divisors allow it (decided once per workgroup)."""
+            if not dv:
+                return call.format(S=f"xg_s{sid}") + ";"
+            return (f"if (xg_s{sid}::dv_ok({rv}, {pv})) {call.format(S=f'xg_s{sid}_r')}; "
+                    f"else {call.format(S=f'xg_s{sid}')};")
+
         args = f"const xg_launch_hdr H, const xg_r{sid} R, const xg_p{sid} P"
         remap = "xg::xcd_remap(blockIdx.x, gridDim.x)"
         # A staged 3-D statement's kernel holds plane registers and an LDS copy: ask for the
@@ -645,7 +689,7 @@ class Emitter:
             wv = f" __attribute__((amdgpu_waves_per_eu({w})))"
             wvb = f" __attribute__((amdgpu_waves_per_eu({max(1, w - 2)})))"
         out.append(f'extern "C" __global__ void __launch_bounds__(256){wv} xg_k{sid}({args}) '
-                   f"{{ xg::run_main<xg_s{sid}, XG_OVERSTEP>(H, R, P, {remap}, gridDim.x); }}")
+                   f"{{ {disp('xg::run_main<{S}, XG_OVERSTEP>(H, R, P, ' + remap + ', gridDim.x)', 'R', 'P')} }}")
         # batched: one launch over the tiles of many instances (entries in a device table)
         # The entry's header, views and parameters are copied into locals first: read
         # through the table's generic pointers they could alias the kernel's own
@@ -655,13 +699,13 @@ class Emitter:
                  f"const xg_p{sid} p = *xg::cp<xg_p{sid}>(e.params);")
         out.append(f'extern "C" __global__ void __launch_bounds__(256){wvb} xg_k{sid}_b({bargs}) '
                    f"{{ xg::for_batch_tiles(E, n, total, [&](const xg_bent& e, uint32_t t) "
-                   f"{{ {local} xg::run_main<xg_s{sid}, XG_OVERSTEP>(h, r, p, t, e.ntiles); }}); }}")
+                   f"{{ {local} {disp('xg::run_main<{S}, XG_OVERSTEP>(h, r, p, t, e.ntiles)', 'r', 'p')} }}); }}")
         if st.mode == "pingpong":
             out.append(f'extern "C" __global__ void __launch_bounds__(256) xg_k{sid}_pp({args}) '
-                       f"{{ xg::run_pingpong<xg_s{sid}, XG_OVERSTEP>(H, R, P, {remap}); }}")
+                       f"{{ {disp('xg::run_pingpong<{S}, XG_OVERSTEP>(H, R, P, ' + remap + ')', 'R', 'P')} }}")
             out.append(f'extern "C" __global__ void __launch_bounds__(256) xg_k{sid}_ppb({bargs}) '
                        f"{{ xg::for_batch_tiles(E, n, total, [&](const xg_bent& e, uint32_t t) "
-                       f"{{ {local} xg::run_pingpong<xg_s{sid}, XG_OVERSTEP>(h, r, p, t); }}); }}")
+                       f"{{ {local} {disp('xg::run_pingpong<{S}, XG_OVERSTEP>(h, r, p, t)', 'r', 'p')} }}); }}")
 
     def _device_function(self, op, out: list):
         f = op.ir
@@ -753,11 +797,13 @@ class Emitter:
         o(f"  constexpr int RE = {L.reach};")
         o("  int cb = 0;                 /* offset of the current copy in buf / vl */")
         # one instantiation per addressing mode of the other grids, chosen once per workgroup
-        o("  auto iterate = [&](auto safe_c) {")
+        o("  auto iterate = [&](auto safe_c, auto dv_c) {")
         o("  constexpr bool SAFE = decltype(safe_c)::value;")
+        o("  constexpr bool DV = decltype(dv_c)::value;   /* reciprocal form of uniform divisions */")
+        o("  (void)DV;")
         for j, s in enumerate(L.stmts):
             if s.stmt.sid in self._prefetched:
-                S = f"xg_s{s.stmt.sid}"
+                S = f"xg_s{s.stmt.sid}_t<DV>"
                 o(f"  typename {S}::Pre pre{j}[NC];")
                 o("#pragma unroll")
                 o("  for (int k = 0; k < NC; ++k) {")
@@ -767,7 +813,7 @@ class Emitter:
                 o("  }")
         o("  for (int it = 0; it < TH.iters; ++it) {")
         for j, s in enumerate(L.stmts):
-            S = f"xg_s{s.stmt.sid}"
+            S = f"xg_s{s.stmt.sid}_t<DV>"
             bit = f"{1 << j}u"
             if s.stmt.klass != 0:
                 # boundary statement: its listed cells into the spare copy, then back
@@ -843,7 +889,7 @@ class Emitter:
             o("    }")
         o("  }")
         o("  };")
-        o("  if (safe) iterate(xg::bconst<true>()); else iterate(xg::bconst<false>());")
+        self._tb_iterate_call(L, o)
         o("  const T* lds = buf + cb;")
         o("  const uint8_t* vld = vl + (CHECK ? cb : 0);")
         o("  const xg::gptr<T> __restrict__ dst = xg::gp<T>(TH.dst.ptr);")
@@ -868,6 +914,19 @@ class Emitter:
           f"{{ {L.kernel}_body<true>({call}); }}")
         if L.batched:
             self._tb_batched(L, out)
+
+    def _tb_iterate_call(self, L: LoopMeta, o) -> None:
+        """Run a blocked body's iterations in the instantiation for this workgroup:
+        safe (no clamped loads) or not, and with the reciprocal form of uniform
+        divisions when every statement's divisors allow it."""
+        conds = [f"xg_s{s.stmt.sid}::dv_ok(R{j}, P{j})" for j, s in enumerate(L.stmts) if self._dv.get(s.stmt.sid)]
+        t, f = "xg::bconst<true>()", "xg::bconst<false>()"
+        if not conds:
+            o(f"  if (safe) iterate({t}, {f}); else iterate({f}, {f});")
+            return
+        o(f"  const bool dv = {' && '.join(conds)};")
+        o(f"  if (safe) {{ if (dv) iterate({t}, {t}); else iterate({t}, {f}); }}")
+        o(f"  else {{ if (dv) iterate({f}, {t}); else iterate({f}, {f}); }}")
 
     @staticmethod
     def _tb_bounds(L: LoopMeta, check: bool = False) -> str:
@@ -1002,11 +1061,13 @@ class Emitter:
         o("    }")
         o("    xb ^= 1;")
         o("  };")
-        o("  auto iterate = [&](auto safe_c) {")
+        o("  auto iterate = [&](auto safe_c, auto dv_c) {")
         o("  constexpr bool SAFE = decltype(safe_c)::value;")
+        o("  constexpr bool DV = decltype(dv_c)::value;   /* reciprocal form of uniform divisions */")
+        o("  (void)DV;")
         for j, s_ in enumerate(L.stmts):
             if s_.stmt.sid in self._prefetched:
-                S = f"xg_s{s_.stmt.sid}"
+                S = f"xg_s{s_.stmt.sid}_t<DV>"
                 o(f"  typename {S}::Pre pre{j}[BR];")
                 o("#pragma unroll")
                 o("  for (int q = 0; q < BR; ++q)")
@@ -1014,7 +1075,7 @@ class Emitter:
                   f"{S}::template prefetch<SAFE>(pre{j}[q], R{j}, sh, r0 + br + q, cg, 0);")
         o("  for (int it = 0; it < TH.iters; ++it) {")
         for j, s_ in enumerate(L.stmts):
-            S = f"xg_s{s_.stmt.sid}"
+            S = f"xg_s{s_.stmt.sid}_t<DV>"
             bit = f"{1 << j}u"
             reads = self._tb_reads[s_.stmt.sid]
             vertical = any(r[0] != 0 for r in reads)
@@ -1064,7 +1125,7 @@ class Emitter:
             o("    }")
         o("  }")
         o("  };")
-        o("  if (safe) iterate(xg::bconst<true>()); else iterate(xg::bconst<false>());")
+        self._tb_iterate_call(L, o)
         o("  const xg::gptr<T> __restrict__ dst = xg::gp<T>(TH.dst.ptr);")
         o("#pragma unroll")
         o("  for (int q = 0; q < BR; ++q) {")
@@ -1103,12 +1164,16 @@ class Emitter:
         for m, nxt in zip(stmt_meta, stmt_meta[1:]):
             if m.pair:
                 a, b = m.stmt.sid, nxt.stmt.sid
+                fw = 'true' if m.pair_fwd else 'false'
+                run = (f"xg::run_pair<xg_s{a}{{s}}, xg_s{b}{{s}}, XG_OVERSTEP, {fw}>(HA, RA, PA, HB, RB, PB, "
+                       f"xg::xcd_remap(blockIdx.x, gridDim.x));")
+                conds = [f"xg_s{x}::dv_ok(R{y}, P{y})" for x, y in ((a, "A"), (b, "B")) if self._dv.get(x)]
+                body = (f"if ({' && '.join(conds)}) {run.format(s='_r')} else {run.format(s='')}" if conds
+                        else run.format(s=""))
                 out.append(f'extern "C" __global__ void __launch_bounds__(256) {m.pair}('
                            f"const xg_launch_hdr HA, const xg_r{a} RA, const xg_p{a} PA, "
                            f"const xg_launch_hdr HB, const xg_r{b} RB, const xg_p{b} PB) "
-                           f"{{ xg::run_pair<xg_s{a}, xg_s{b}, XG_OVERSTEP, {'true' if m.pair_fwd else 'false'}>"
-                           f"(HA, RA, PA, HB, RB, PB, "
-                           f"xg::xcd_remap(blockIdx.x, gridDim.x)); }}")
+                           f"{{ {body} }}")
         for L in loops:
             for V in (L, L.alt):
                 if V is not None:
