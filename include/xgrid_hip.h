@@ -231,6 +231,17 @@ int xg_batch_counters(xg_batch* b, int64_t* launches, int64_t* replays, int64_t*
  * Synchronises the backend stream. */
 int xg_grid_norm_diff(xg_grid* a, int32_t level_a, int32_t type_a, xg_grid* b, int32_t level_b, int32_t type_b,
                       int32_t klass, double* l2, double* linf, int64_t* cells);
+/* No counterpart in the reference.  For callers that cannot allocate (tight
+ * serving loops, harnesses that forbid hipMalloc): xg_grid_norm_diff with the
+ * same arguments and results, but it allocates nothing, not even on its first
+ * call.  Its partials live in a static device array of 1024 partials + 1
+ * result (16,400 bytes per GPU, held until the process ends; its address is
+ * looked up by xg_init).  Any class size fits: at most 1024 workgroups each
+ * reduce a strided slice.  Every call on a GPU shares that GPU's array, so
+ * calls that overlap on one GPU (several host threads at once) race.
+ * Synchronises the backend stream. */
+int xg_grid_norm_diff_noalloc(xg_grid* a, int32_t level_a, int32_t type_a, xg_grid* b, int32_t level_b,
+                              int32_t type_b, int32_t klass, double* l2, double* linf, int64_t* cells);
 /* Norms of the residual r = sum_a coef[a] (p[+e_a] - 2p + p[-e_a]) - rhs_scale * rhs
  * of a Poisson equation over the cells of class `klass` of p, every cell of
  * which must lie at least one cell inside the grid (else XG_ERR_ARG).  coef

@@ -110,6 +110,8 @@ _SIGS = {
                            POINTER(c_int64)], c_int),
     "xg_grid_norm_diff": ([c_void_p, c_int32, c_int32, c_void_p, c_int32, c_int32, c_int32, POINTER(c_double),
                            POINTER(c_double), POINTER(c_int64)], c_int),
+    "xg_grid_norm_diff_noalloc": ([c_void_p, c_int32, c_int32, c_void_p, c_int32, c_int32, c_int32, POINTER(c_double),
+                                   POINTER(c_double), POINTER(c_int64)], c_int),
     "xg_grid_poisson_residual": ([c_void_p, c_int32, c_int32, c_void_p, c_int32, c_int32, c_int32, POINTER(c_double),
                                   c_double, POINTER(c_double), POINTER(c_double)], c_int),
 }

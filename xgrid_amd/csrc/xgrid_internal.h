@@ -50,6 +50,10 @@ inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 // hipMalloc + zero fill on the backend stream
 int alloc_zero(size_t bytes, void** out);
 
+// Look up this device's address of the reductions' static scratch (device
+// `device` current); called once per device by xg_init.
+int reduce_setup(int device);
+
 }  // namespace xgi
 
 // Device plan of one boundary class of a grid.

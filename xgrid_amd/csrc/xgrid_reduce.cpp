@@ -143,6 +143,16 @@ __global__ void __launch_bounds__(kRedThreads) final_reduce(const Partial* part,
 
 Partial* g_part = nullptr;   // kRedBlocks partials + 1 result, kept for the process
 
+// Scratch of the allocation-free entry point (xg_grid_norm_diff_noalloc): the
+// same kRedBlocks partials + 1 result, 16,400 bytes, in device memory that
+// every GPU the library runs on holds until the process ends.  In host code
+// `scratch` names a host-side stand-in, not a device address: set-up looks up
+// each device's copy once (reduce_setup, from xg_init) and only that address
+// is passed to kernels.
+static __device__ __align__(256) Partial scratch[kRedBlocks + 1];
+constexpr int kMaxDevices = 64;
+Partial* g_scratch[kMaxDevices] = {};
+
 int ensure_partials() {
   if (g_part) return XG_OK;
   void* p = nullptr;
@@ -177,13 +187,13 @@ int class_domain(const xg_grid* G, int32_t klass, xg_dom& dom, int32_t* shape3, 
   return XG_OK;
 }
 
-int finish(int64_t total, double* l2, double* linf) {
+int finish(int64_t total, double* l2, double* linf, Partial* part) {
   Partial h{0.0, 0.0};
   if (total > 0) {
     const int nb = (int)std::min<int64_t>(kRedBlocks, ceil_div(total, kRedThreads));
-    hipLaunchKernelGGL(final_reduce, dim3(1), dim3(kRedThreads), 0, g_dev.stream, g_part, nb, g_part + kRedBlocks);
+    hipLaunchKernelGGL(final_reduce, dim3(1), dim3(kRedThreads), 0, g_dev.stream, part, nb, part + kRedBlocks);
     XG_HIP(hipGetLastError());
-    XG_HIP(hipMemcpyAsync(&h, g_part + kRedBlocks, sizeof h, hipMemcpyDeviceToHost, g_dev.stream));
+    XG_HIP(hipMemcpyAsync(&h, part + kRedBlocks, sizeof h, hipMemcpyDeviceToHost, g_dev.stream));
     XG_HIP(hipStreamSynchronize(g_dev.stream));
   }
   if (l2) *l2 = std::sqrt(h.sum);
@@ -193,13 +203,11 @@ int finish(int64_t total, double* l2, double* linf) {
 
 bool float_type(int32_t t) { return t == XG_T_F32 || t == XG_T_F64; }
 
-}  // namespace
-
-extern "C" {
-
-int xg_grid_norm_diff(xg_grid* a, int32_t level_a, int32_t type_a, xg_grid* b, int32_t level_b, int32_t type_b,
-                      int32_t klass, double* l2, double* linf, int64_t* cells) {
-  if (int rc = check_ready()) return rc;
+// L2 / Linf of a - b over class klass of a, reducing through `part` (room for
+// kRedBlocks partials + 1 result).  Every partial the final pass reads is
+// written by the same call first, so nothing in `part` needs clearing.
+int norm_diff(xg_grid* a, int32_t level_a, int32_t type_a, xg_grid* b, int32_t level_b, int32_t type_b,
+              int32_t klass, double* l2, double* linf, int64_t* cells, Partial* part) {
   if (!a || level_a < 0 || level_a >= (int32_t)a->levels.size() || !float_type(type_a))
     return fail(XG_ERR_ARG, "invalid first grid, level or element type");
   if (b && (level_b < 0 || level_b >= (int32_t)b->levels.size() || !float_type(type_b)))
@@ -208,7 +216,6 @@ int xg_grid_norm_diff(xg_grid* a, int32_t level_a, int32_t type_a, xg_grid* b, i
     return fail(XG_ERR_ARG, "grids of different shapes");
   if ((int64_t)a->elem != (type_a == XG_T_F32 ? 4 : 8) || (b && (int64_t)b->elem != (type_b == XG_T_F32 ? 4 : 8)))
     return fail(XG_ERR_ARG, "element type does not match the grid's element size");
-  if (int rc = ensure_partials()) return rc;
   xg_dom dom;
   int32_t sh[3];
   int64_t total = 0;
@@ -221,10 +228,45 @@ int xg_grid_norm_diff(xg_grid* a, int32_t level_a, int32_t type_a, xg_grid* b, i
     const bool fa = type_a == XG_T_F32, fb = !b || type_b == XG_T_F32;
     auto* k = fa ? (fb ? diff_partials<float, float> : diff_partials<float, double>)
                  : (fb ? diff_partials<double, float> : diff_partials<double, double>);
-    hipLaunchKernelGGL(k, dim3(nb), dim3(kRedThreads), 0, g_dev.stream, pa, pb, dom, sh[0], sh[1], sh[2], total, g_part);
+    hipLaunchKernelGGL(k, dim3(nb), dim3(kRedThreads), 0, g_dev.stream, pa, pb, dom, sh[0], sh[1], sh[2], total, part);
     XG_HIP(hipGetLastError());
   }
-  return finish(total, l2, linf);
+  return finish(total, l2, linf, part);
+}
+
+}  // namespace
+
+namespace xgi {
+
+int reduce_setup(int device) {
+  if (device < 0 || device >= kMaxDevices) return fail(XG_ERR_ARG, "device index beyond the scratch table");
+  if (g_scratch[device]) return XG_OK;
+  void* p = nullptr;
+  XG_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  g_scratch[device] = (Partial*)p;
+  return XG_OK;
+}
+
+}  // namespace xgi
+
+extern "C" {
+
+int xg_grid_norm_diff(xg_grid* a, int32_t level_a, int32_t type_a, xg_grid* b, int32_t level_b, int32_t type_b,
+                      int32_t klass, double* l2, double* linf, int64_t* cells) {
+  if (int rc = check_ready()) return rc;
+  if (int rc = ensure_partials()) return rc;
+  return norm_diff(a, level_a, type_a, b, level_b, type_b, klass, l2, linf, cells, g_part);
+}
+
+int xg_grid_norm_diff_noalloc(xg_grid* a, int32_t level_a, int32_t type_a, xg_grid* b, int32_t level_b,
+                              int32_t type_b, int32_t klass, double* l2, double* linf, int64_t* cells) {
+  if (int rc = check_ready()) return rc;
+  // The partials fit whatever the class size: at most kRedBlocks workgroups, each
+  // reducing a strided slice of the cells.
+  static_assert(sizeof scratch / sizeof(Partial) == kRedBlocks + 1, "scratch holds every partial and the result");
+  Partial* part = g_dev.id >= 0 && g_dev.id < kMaxDevices ? g_scratch[g_dev.id] : nullptr;
+  if (!part) return fail(XG_ERR_STATE, "the device's scratch address was not looked up (xg_init)");
+  return norm_diff(a, level_a, type_a, b, level_b, type_b, klass, l2, linf, cells, part);
 }
 
 int xg_grid_poisson_residual(xg_grid* p, int32_t level_p, int32_t type_p, xg_grid* rhs, int32_t level_rhs,
@@ -260,7 +302,7 @@ int xg_grid_poisson_residual(xg_grid* p, int32_t level_p, int32_t type_p, xg_gri
                        total, g_part);
     XG_HIP(hipGetLastError());
   }
-  return finish(total, l2, linf);
+  return finish(total, l2, linf, g_part);
 }
 
 }  // extern "C"

@@ -817,6 +817,7 @@ int xg_init(int device) {
   if (e != hipSuccess || n == 0) return fail(XG_ERR_HIP, "no HIP device available");
   if (device < 0 || device >= n) return fail(XG_ERR_ARG, "device index out of range");
   XG_HIP(hipSetDevice(device));
+  if (int rc = reduce_setup(device)) return rc;
   hipDeviceProp_t prop;
   XG_HIP(hipGetDeviceProperties(&prop, device));
   XG_HIP(hipStreamCreateWithFlags(&g_dev.stream, hipStreamNonBlocking));
