@@ -209,6 +209,7 @@ class HipKernel:
 
     def _param_size(self, m) -> int:
         fields_ = [(v.name, (v.type.element if isinstance(v.type, Pointer) else v.type).ctype) for v in m.params]
+        fields_ += [(h.name, ctypes.c_double if h.ctype == "double" else ctypes.c_float) for h in m.hoisted]
         if not fields_:
             fields_ = [("xg_unused", ctypes.c_int32)]
         return ctypes.sizeof(type("P", (ctypes.Structure,), {"_fields_": fields_}))

@@ -85,6 +85,20 @@ class StmtMeta:
     pair: str | None = None       # kernel running this statement and the next in one launch
     pair_fwd: bool = False        # ... with the next one's loads of this store (own cell, time 0) forwarded
     fwd_from: str | None = None   # (second of a forwarding pair) the grid whose time-0 loads are forwarded
+    hoisted: list = field(default_factory=list)   # [Hoist]: launch-uniform subexpressions computed by the host
+
+
+@dataclass
+class Hoist:
+    """A launch-uniform subexpression of a statement (no grid load): the host stub
+    evaluates it once per call and passes the value in the parameter block, field
+    `name`, so the device does not evaluate it for every cell.  The host's C
+    evaluates the same operations on the same values in the same order (IEEE,
+    no contraction), so the value is the one the device would have computed."""
+    nodes: list                   # the IR nodes with this text
+    ctype: str                    # "double" | "float"
+    name: str                     # parameter field
+    host: str                     # C text the stub evaluates
 
 
 @dataclass
@@ -129,7 +143,8 @@ class Lowered:
 class _ExprC:
     """Expression text with C semantics, shared by host and device lowering."""
 
-    def __init__(self, device: bool, cell=None, name=None, grid_info=None, udivs: list | None = None):
+    def __init__(self, device: bool, cell=None, name=None, grid_info=None, udivs: list | None = None,
+                 hoisted: dict | None = None, fold_pow: bool = False):
         self.device = device
         self.cell = cell              # callback for ir.Cell (device statements only)
         self.name = name              # callback for ir.Name
@@ -137,8 +152,13 @@ class _ExprC:
         # device statements: divisor texts of the per-cell divisions by a launch-uniform
         # value, which are emitted as xg::udiv<DV> (see _uniform_division)
         self.udivs = udivs
+        self.hoisted = hoisted or {}  # id(node) -> text of its host-computed value
+        self.fold_pow = fold_pow      # host text with the device's pow folds (hoisted values)
 
     def __call__(self, e) -> str:
+        h = self.hoisted.get(id(e))
+        if h is not None:
+            return h
         m = getattr(self, "e_" + type(e).__name__, None)
         if m is None:
             raise EmitError(f"cannot lower expression {type(e).__name__}")
@@ -162,12 +182,12 @@ class _ExprC:
         if e.op == "**":
             double = isinstance(e.type, Floating) and e.type.width_bytes == 8
             fp = "double" if double else "float"
-            if not self.device:
+            if not self.device and not self.fold_pow:
                 return f"{'pow' if double else 'powf'}({self(e.left)}, {self(e.right)})"
             fold = _pow_fold(e.right)
             x = f"(({fp})({self(e.left)}))"
             if fold == "sq":
-                return f"xg::sq{x}"
+                return f"xg::sq{x}" if self.device else f"({x} * {x})"
             if fold == "id":
                 return x
             if fold == "inv":
@@ -196,12 +216,107 @@ class _ExprC:
             if self.device:
                 return f"{e.ctor.name}{{{', '.join(args)}}}"
             return f"(({ctype_name(e.ctor)}) {{{', '.join(args)}}})"
+        if self.device and is_device_math(e.callee):
+            ptypes = DEVICE_MATH[e.callee.name][1]
+            if len(ptypes) != len(args):
+                raise EmitError(f"{e.loc}: '{e.callee.name}' takes {len(ptypes)} argument(s)")
+            args = [f"(({t})({a}))" for t, a in zip(ptypes, args)]
         return f"{function_symbol(e.callee)}({', '.join(args)})"
 
     def e_GridInfo(self, e):
         if e.info == "dimension":
             return str(e.grid.type.dimension)
         return self.grid_info(e)
+
+
+_FP_RANK = {"float": 1, "double": 2}
+
+
+def _c_type(e):
+    """The C type of an expression's text as this module writes it (C's usual
+    arithmetic conversions; an unsuffixed float literal is a double, as in the
+    reference's generated C), or None where it is not a plain scalar."""
+    if isinstance(e, ir.Const):
+        if type(e.value) is float:
+            return "double"
+        if type(e.value) is bool:
+            return "int"
+        return "int" if abs(e.value) < 2 ** 31 else None
+    if isinstance(e, (ir.Name, ir.Field)):
+        t = e.type
+        if isinstance(t, Floating):
+            return "double" if t.width_bytes == 8 else "float"
+        return "int" if isinstance(t, (Integer, Boolean)) else None
+    if isinstance(e, ir.GridInfo):
+        return "int"
+    if isinstance(e, ir.Cast):
+        t = e.type
+        if isinstance(t, Floating):
+            return "double" if t.width_bytes == 8 else "float"
+        return "int" if isinstance(t, (Integer, Boolean)) else None
+    if isinstance(e, ir.Un):
+        return _c_type(e.operand)
+    if isinstance(e, ir.Bin):
+        if e.op == "**":
+            return "double" if isinstance(e.type, Floating) and e.type.width_bytes == 8 else "float"
+        if e.op in ("==", "!=", "<", "<=", ">", ">=", "&&", "||"):
+            return "int"
+        a, b = _c_type(e.left), _c_type(e.right)
+        if a is None or b is None:
+            return None
+        ra, rb = _FP_RANK.get(a, 0), _FP_RANK.get(b, 0)
+        return a if ra >= rb else b
+    if isinstance(e, ir.Select):
+        a, b = _c_type(e.then), _c_type(e.orelse)
+        if a is None or b is None:
+            return None
+        return a if _FP_RANK.get(a, 0) >= _FP_RANK.get(b, 0) else b
+    return None
+
+
+def _host_exact(e) -> bool:
+    """Can the host stub compute this subexpression, bit for bit as the device
+    would, once per call: no grid load, no call (device and host functions may
+    differ), no integer division or remainder (division by zero would stop the
+    host), no conversion of a floating value to an integer (undefined out of
+    range), and `**` only where both sides fold it (x*x, x, 1/x)."""
+    if isinstance(e, ir.Const):
+        return _c_type(e) is not None
+    if isinstance(e, ir.Name):
+        return not isinstance(e.type, Structure)
+    if isinstance(e, ir.Field):
+        return _host_exact(e.value) if not isinstance(e.value, ir.Name) else True
+    if isinstance(e, ir.GridInfo):
+        return e.dim is None or isinstance(e.dim, ir.Const)
+    if isinstance(e, ir.Cast):
+        src = _c_type(e.value)
+        if src in _FP_RANK and not isinstance(e.type, Floating):
+            return False
+        return src is not None and _host_exact(e.value)
+    if isinstance(e, ir.Un):
+        return _host_exact(e.operand)
+    if isinstance(e, ir.Select):
+        return _host_exact(e.cond) and _host_exact(e.then) and _host_exact(e.orelse)
+    if isinstance(e, ir.Bin):
+        if e.op == "**" and _pow_fold(e.right) is None:
+            return False
+        if e.op in ("/", "%") and not isinstance(e.type, Floating):
+            return False
+        return _host_exact(e.left) and (e.op == "**" or _host_exact(e.right))
+    return False
+
+
+def param_fields(m, type_name) -> str:
+    """Fields of a statement's parameter block: its captured scalars, then its
+    hoisted values (the host stub, the device code and ctypes share this layout)."""
+    parts = [f"{type_name(captured_value_type(v))} {v.name};" for v in m.params]
+    parts += [f"{h.ctype} {h.name};" for h in m.hoisted]
+    return " ".join(parts) if parts else "int32_t xg_unused;"
+
+
+def _is_leaf(e) -> bool:
+    return isinstance(e, (ir.Const, ir.Name, ir.GridInfo)) or \
+        (isinstance(e, ir.Field) and _is_leaf(e.value))
 
 
 def _uniform_division(e) -> bool:
@@ -215,6 +330,31 @@ def _uniform_division(e) -> bool:
         return False
     return any(isinstance(n, ir.Cell) for n in ir.walk(e.left)) and \
         not any(isinstance(n, ir.Cell) for n in ir.walk(e.right))
+
+
+def _math_table() -> dict:
+    """C99 <math.h> functions the device provides under the same name: name ->
+    (return type, parameter types).  An @xgrid.external of one of these may be
+    called inside a stencil statement; its arguments are converted to the C
+    prototype's types first, so the device evaluates what the reference's C
+    evaluates (a float argument of `sin` is widened to double, as in C)."""
+    one = ["sqrt", "cbrt", "fabs", "floor", "ceil", "trunc", "round", "exp", "exp2", "expm1", "log", "log2",
+           "log10", "log1p", "sin", "cos", "tan", "asin", "acos", "atan", "sinh", "cosh", "tanh", "asinh",
+           "acosh", "atanh", "erf", "erfc", "tgamma", "lgamma"]
+    two = ["pow", "atan2", "fmod", "fmin", "fmax", "hypot", "copysign", "fdim"]
+    out = {}
+    for names, k in ((one, 1), (two, 2), (["fma"], 3)):
+        for n in names:
+            out[n] = ("double", ["double"] * k)
+            out[n + "f"] = ("float", ["float"] * k)
+    return out
+
+
+DEVICE_MATH = _math_table()
+
+
+def is_device_math(op) -> bool:
+    return op.mode == "external" and op.name in DEVICE_MATH
 
 
 def function_symbol(op) -> str:
@@ -443,9 +583,13 @@ class Emitter:
             for n in ir.walk(st.node.value):
                 if isinstance(n, ir.Call) and n.callee is not None:
                     visit_fn(n.callee, self.device_functions, dseen)
+        # externals in stencil statements: the <math.h> functions the device has
         for op in self.device_functions:
-            if op.mode == "external":
-                raise EmitError(f"external function '{op.name}' cannot be evaluated inside a stencil statement")
+            if op.mode == "external" and not is_device_math(op):
+                raise EmitError(f"external function '{op.name}' cannot be evaluated inside a stencil statement: "
+                                f"only <math.h> functions the device provides can ({', '.join(sorted(DEVICE_MATH))})")
+        self.device_functions = [op for op in self.device_functions if not is_device_math(op)]
+        for op in self.device_functions:
             for n in ir.walk_all(op.ir.body):
                 if isinstance(n, ir.Inline):
                     raise EmitError(f"@function '{op.name}' with inline C cannot be evaluated inside a stencil statement")
@@ -482,9 +626,11 @@ class Emitter:
         return plan if plan.usable() else None
 
     def _device_stmt(self, st: StencilStmt, refs: list, params: list, out: list, tb: LoopMeta | None = None,
-                     fwd_from: str | None = None):
+                     fwd_from: str | None = None, meta: StmtMeta | None = None):
         sid = st.sid
         D = st.dimension
+        # launch-uniform subexpressions the host computed: read from the parameter block
+        hoisted = {id(n): f"P.{h.name}" for h in (meta.hoisted if meta else []) for n in h.nodes}
         T = device_type_name(st.gtype.element)
         grids = {n: t for _, n, t in self.func.grid_args}
         in_lds = [False]     # while lowering value_tb: the loop grid's level 0 lives in LDS
@@ -531,28 +677,28 @@ class Emitter:
             return f"R.v[{r}].shape[{_ExprC(True, cell, name, grid_info)(e.dim)}]"
 
         udivs: list = []      # divisor texts of the per-cell divisions by a launch-uniform value
-        expr = _ExprC(True, cell, name, grid_info, udivs)(st.node.value)
+        expr = _ExprC(True, cell, name, grid_info, udivs, hoisted)(st.node.value)
         expr_st3 = None
         plan3 = self._plan_st3(st, refs, grids) if D == 3 and tb is None else None
         if plan3 is not None:
             st3[0] = plan3
-            expr_st3 = _ExprC(True, cell, name, grid_info, udivs)(st.node.value)
+            expr_st3 = _ExprC(True, cell, name, grid_info, udivs, hoisted)(st.node.value)
             st3[0] = None
         expr_tb = expr_pre = expr_lds = expr_lds_pre = None
         if tb is not None:
             in_reg[0] = True
-            expr_tb = _ExprC(True, cell, name, grid_info, udivs)(st.node.value)
+            expr_tb = _ExprC(True, cell, name, grid_info, udivs, hoisted)(st.node.value)
             in_reg[0] = False
             in_lds[0] = True
-            expr_lds = _ExprC(True, cell, name, grid_info, udivs)(st.node.value)
+            expr_lds = _ExprC(True, cell, name, grid_info, udivs, hoisted)(st.node.value)
             in_lds[0] = False
             if st.klass == 0:
                 pre_mode[0] = True
                 in_reg[0] = True
-                expr_pre = _ExprC(True, cell, name, grid_info, udivs)(st.node.value)
+                expr_pre = _ExprC(True, cell, name, grid_info, udivs, hoisted)(st.node.value)
                 in_reg[0] = False
                 in_lds[0] = True
-                expr_lds_pre = _ExprC(True, cell, name, grid_info, udivs)(st.node.value)
+                expr_lds_pre = _ExprC(True, cell, name, grid_info, udivs, hoisted)(st.node.value)
                 in_lds[0] = False
                 pre_mode[0] = False
                 if len(pre_slots) > self.PREFETCH_MAX:
@@ -567,11 +713,7 @@ class Emitter:
                     rh[d] = max(rh[d], o)
         nr = max(1, len(refs))
         out.append(f"// statement {sid}: {st.node.loc}")
-        if params:
-            fields_ = " ".join(f"{device_type_name(captured_value_type(v))} {v.name};" for v in params)
-            out.append(f"struct xg_p{sid} {{ {fields_} }};")
-        else:
-            out.append(f"struct xg_p{sid} {{ int32_t xg_unused; }};")
+        out.append(f"struct xg_p{sid} {{ {param_fields(meta, device_type_name)} }};")
         out.append(f"struct xg_r{sid} {{ xg_gview v[{nr}]; }};")
         # DV: the statement's uniform divisors were found to be exact powers of two for
         # this launch (dv_ok), so xg::udiv multiplies by their reciprocals
@@ -601,7 +743,7 @@ class Emitter:
             # the same expression with its time-0 loads of `fwd_from` at the cell taken from the
             # value the paired statement just computed for that cell
             fwd[0] = True
-            expr_fwd = _ExprC(True, cell, name, grid_info, udivs)(st.node.value)
+            expr_fwd = _ExprC(True, cell, name, grid_info, udivs, hoisted)(st.node.value)
             fwd[0] = False
             ft = device_type_name(grids[fwd_from].element)
             out.append("  template <bool SAFE>")
@@ -738,7 +880,7 @@ class Emitter:
                         rl[d] = max(rl[d], -n.offsets[d])
                         rh[d] = max(rh[d], n.offsets[d])
         o = out.append
-        o(f"template <bool CHECK>")
+        o(f"template <bool CHECK, bool DVOK>")
         o(f"__device__ __forceinline__ void {L.kernel}_body({', '.join(params)}, const int tile) {{")
         o(f"  typedef {T} T;")
         o(f"  constexpr int TBT = {L.threads};")
@@ -909,9 +1051,9 @@ class Emitter:
                                           for j, s in enumerate(L.stmts)]
         call = ", ".join(["TH"] + [f"R{j}, P{j}" for j in range(m)] + ["(int)xg::xcd_remap(blockIdx.x, gridDim.x)"])
         o(f'extern "C" __global__ void {self._tb_bounds(L)} {L.kernel}({", ".join(kargs)}) '
-          f"{{ {L.kernel}_body<false>({call}); }}")
+          f"{{ {L.kernel}_body<false, true>({call}); }}")
         o(f'extern "C" __global__ void {self._tb_bounds(L, True)} {L.kernel}_chk({", ".join(kargs)}) '
-          f"{{ {L.kernel}_body<true>({call}); }}")
+          f"{{ {L.kernel}_body<true, true>({call}); }}")
         if L.batched:
             self._tb_batched(L, out)
 
@@ -924,9 +1066,19 @@ class Emitter:
         if not conds:
             o(f"  if (safe) iterate({t}, {f}); else iterate({f}, {f});")
             return
-        o(f"  const bool dv = {' && '.join(conds)};")
-        o(f"  if (safe) {{ if (dv) iterate({t}, {t}); else iterate({t}, {f}); }}")
-        o(f"  else {{ if (dv) iterate({f}, {t}); else iterate({f}, {f}); }}")
+        # DVOK: the batched variants leave the reciprocal form out (their instances
+        # seldom have power-of-two divisors, and its second instantiation of the
+        # iterations made the batched LDS kernel spill 11 registers instead of 3)
+        # Edge tiles (clamped loads) take the reciprocal form too: the 2049^2 cavity step
+        # measured 0.648 ms against 0.676 ms when they divided (FP32 fields: 0.506
+        # against 0.550), although FP32's register budget then spills
+        o("  if constexpr (DVOK) {")
+        o(f"    const bool dv = {' && '.join(conds)};")
+        o(f"    if (safe) {{ if (dv) iterate({t}, {t}); else iterate({t}, {f}); }}")
+        o(f"    else {{ if (dv) iterate({f}, {t}); else iterate({f}, {f}); }}")
+        o("  } else {")
+        o(f"    if (safe) iterate({t}, {f}); else iterate({f}, {f});")
+        o("  }")
 
     @staticmethod
     def _tb_bounds(L: LoopMeta, check: bool = False) -> str:
@@ -969,7 +1121,7 @@ class Emitter:
             out.append(f'extern "C" __global__ void {self._tb_bounds(L, chk == "true")} {L.kernel}{suffix}('
                        f"const xg_btent* __restrict__ E, const int32_t n, const int64_t total) "
                        f"{{ xg::for_batch_tiles(E, n, total, [&](const xg_btent& e, uint32_t t) "
-                       f"{{ {' '.join(loc)} {L.kernel}_body<{chk}>({bcall}); __syncthreads(); }}); }}")
+                       f"{{ {' '.join(loc)} {L.kernel}_body<{chk}, false>({bcall}); __syncthreads(); }}); }}")
 
     def _tb_kernel_reg(self, L: LoopMeta, out: list):
         """Blocked kernel of one loop.  A workgroup owns a core tile of the loop grid
@@ -998,7 +1150,7 @@ class Emitter:
                         rl[d] = max(rl[d], -n.offsets[d])
                         rh[d] = max(rh[d], n.offsets[d])
         o = out.append
-        o("template <bool CHECK>")
+        o("template <bool CHECK, bool DVOK>")
         o(f"__device__ __forceinline__ void {L.kernel}_body({', '.join(params)}, const int tile) {{")
         o(f"  typedef {T} T;")
         o(f"  constexpr int TBT = {L.threads}, WV = TBT / 64;")
@@ -1140,9 +1292,9 @@ class Emitter:
                                           for j, s_ in enumerate(L.stmts)]
         call = ", ".join(["TH"] + [f"R{j}, P{j}" for j in range(m)] + ["(int)xg::xcd_remap(blockIdx.x, gridDim.x)"])
         o(f'extern "C" __global__ void {self._tb_bounds(L)} {L.kernel}({", ".join(kargs)}) '
-          f"{{ {L.kernel}_body<false>({call}); }}")
+          f"{{ {L.kernel}_body<false, true>({call}); }}")
         o(f'extern "C" __global__ void {self._tb_bounds(L, True)} {L.kernel}_chk({", ".join(kargs)}) '
-          f"{{ {L.kernel}_body<true>({call}); }}")
+          f"{{ {L.kernel}_body<true, true>({call}); }}")
         if L.batched:
             self._tb_batched(L, out)
 
@@ -1160,7 +1312,7 @@ class Emitter:
         for op in self.device_functions:
             self._device_function(op, out)
         for m in stmt_meta:
-            self._device_stmt(m.stmt, m.refs, m.params, out, loop_of.get(m.stmt.sid), m.fwd_from)
+            self._device_stmt(m.stmt, m.refs, m.params, out, loop_of.get(m.stmt.sid), m.fwd_from, m)
         for m, nxt in zip(stmt_meta, stmt_meta[1:]):
             if m.pair:
                 a, b = m.stmt.sid, nxt.stmt.sid
@@ -1294,6 +1446,8 @@ class Emitter:
         for v in m.params:
             src = f"(*{v.name})" if isinstance(v.type, Pointer) else v.name
             sets.append(f"{var}.{v.name} = {src};")
+        for h in m.hoisted:
+            sets.append(f"{var}.{h.name} = {h.host};")
         return " ".join(sets)
 
     def stub_source(self, stmt_meta: list, loops: list = ()) -> str:
@@ -1332,12 +1486,7 @@ class Emitter:
             self._stmts_text(fi.body, out, self._host_expr(), 1)
             out.append("}")
         for m in stmt_meta:
-            sid = m.stmt.sid
-            if m.params:
-                fields_ = " ".join(f"{ctype_name(captured_value_type(v))} {v.name};" for v in m.params)
-                out.append(f"struct xg_p{sid} {{ {fields_} }};")
-            else:
-                out.append(f"struct xg_p{sid} {{ int32_t xg_unused; }};")
+            out.append(f"struct xg_p{m.stmt.sid} {{ {param_fields(m, ctype_name)} }};")
 
         def stencil(s: ir.Assign) -> str:
             m = next(mm for mm in stmt_meta if mm.stmt.node is s)
@@ -1400,7 +1549,7 @@ class Emitter:
         out.append("size_t xg_args_size(void) { return sizeof(struct xg_args); }")
         return "\n".join(out) + "\n"
 
-    def _host_expr(self):
+    def _host_expr(self, fold_pow: bool = False):
         f = self.func
         grid_names = {n for _, n, _ in f.grid_args}
 
@@ -1414,7 +1563,7 @@ class Emitter:
         def grid_info(e: ir.GridInfo) -> str:
             return f"(xg_A->shape({e.grid.name})[{holder['x'](e.dim)}])"
 
-        x = _ExprC(False, None, name, grid_info)
+        x = _ExprC(False, None, name, grid_info, fold_pow=fold_pow)
         holder["x"] = x
         return x
 
@@ -1431,15 +1580,40 @@ class Emitter:
             stmt_meta.append(StmtMeta(st, refs, list(st.scalars), f"xg_k{st.sid}",
                                       f"xg_k{st.sid}_pp" if st.mode == "pingpong" else None,
                                       phase2_names[key]))
+        for m in stmt_meta:
+            m.hoisted = self._hoists(m.stmt)
         groups = self._groups(stmt_meta)
         self._pairs(stmt_meta)
         loops = self._blockable_loops(stmt_meta)
         graphs_ok = not any(isinstance(n, ir.Inline) for n in ir.walk_all(self.func.body)) and \
-            not any(op.mode == "external" for op in self.functions)
+            not any(op.mode == "external" and not is_device_math(op) for op in self.functions)
         return Lowered(self.device_source(stmt_meta, groups, loops), self.stub_source(stmt_meta, loops), stmt_meta,
                        groups, list(self.func.grid_args),
                        [(i, n, t) for i, (n, t) in enumerate(self.func.args) if not isinstance(t, GridType)],
                        graphs_ok, loops, self.used_shape)
+
+    def _hoists(self, st: StencilStmt) -> list:
+        """The statement's maximal launch-uniform floating-point subexpressions that
+        do some arithmetic (a bare parameter or literal is already a kernel argument),
+        grouped by text.  XGRID_HOIST=0 keeps them on the device (for comparisons)."""
+        if os.environ.get("XGRID_HOIST", "1") == "0":
+            return []
+        host = self._host_expr(fold_pow=True)
+        found: dict = {}
+
+        def visit(e):
+            if not any(isinstance(n, ir.Cell) for n in ir.walk(e)) and not _is_leaf(e) \
+                    and _c_type(e) in _FP_RANK and _host_exact(e):
+                text = host(e)
+                if text not in found:
+                    found[text] = Hoist([], _c_type(e), f"xg_h{len(found)}", text)
+                found[text].nodes.append(e)
+                return
+            for c in ir.children(e):
+                visit(c)
+
+        visit(st.node.value)
+        return list(found.values())
 
     PREFETCH_MAX = 4   # most other-grid loads of a blocked statement kept in registers per cell
 

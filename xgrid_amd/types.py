@@ -13,6 +13,7 @@ from __future__ import annotations
 import ctypes
 import dataclasses
 import struct
+import threading
 import typing
 from dataclasses import dataclass, fields, is_dataclass
 from typing import Any
@@ -141,6 +142,7 @@ class Floating(Number):
 
 
 _STRUCT_CTYPES: dict = {}
+_STRUCT_LOCK = threading.Lock()     # one ctypes type per structure, whichever thread asks first
 
 
 @dataclass(eq=True, frozen=True)
@@ -160,9 +162,11 @@ class Structure(Value):
         key = (self.dataclass, self.name, self.elements)
         ct = _STRUCT_CTYPES.get(key)
         if ct is None:
-            ct = type(f"st{self.name}", (ctypes.Structure,),
-                      {"_fields_": [(n, t.ctype) for n, t in self.elements]})
-            _STRUCT_CTYPES[key] = ct
+            fields_ = [(n, t.ctype) for n, t in self.elements]
+            with _STRUCT_LOCK:
+                ct = _STRUCT_CTYPES.get(key)
+                if ct is None:
+                    ct = _STRUCT_CTYPES[key] = type(f"st{self.name}", (ctypes.Structure,), {"_fields_": fields_})
         return ct
 
     @property
