@@ -12,7 +12,7 @@ from typing import Any
 from .config import init, get_config
 from .grid import Grid
 from .lang import boundary, c
-from .operator import external, function, kernel
+from .operator import external, function, kernel, precompile
 from .types import GridType as _GridType
 from .types import Integer, Void, grid, ptr
 

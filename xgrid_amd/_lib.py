@@ -8,6 +8,7 @@ from __future__ import annotations
 
 import ctypes
 import os
+import threading
 from ctypes import (POINTER, Structure, c_char_p, c_double, c_int, c_int32, c_int64, c_void_p)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -119,12 +120,17 @@ _SIGS = {
 EXPORTED = tuple(_SIGS)
 
 _lib = None
+_load_lock = threading.Lock()
 
 
 def load():
     """Load the library (once) and declare every exported symbol."""
     global _lib
-    if _lib is None:
+    if _lib is not None:
+        return _lib
+    with _load_lock:
+        if _lib is not None:
+            return _lib
         if not os.path.exists(LIB_PATH):
             raise BackendError(f"{LIB_PATH} is missing: build it with `python -m xgrid_amd.build_native`")
         lib = ctypes.CDLL(LIB_PATH, mode=ctypes.RTLD_GLOBAL)

@@ -11,6 +11,7 @@ from __future__ import annotations
 import logging
 import os
 import sys
+import threading
 from dataclasses import asdict, dataclass, field
 from typing import Literal
 
@@ -37,6 +38,7 @@ class Configuration:
     graphs: bool = True            # replay repeated calls as hipGraphs
     fuse: bool = True              # chain small boundary statements in one workgroup; run statement pairs
     blocked: bool = True           # run blockable stencil loops temporally blocked
+    precompile: bool = True        # start compiling a kernel in the background when it is decorated
 
     def __repr__(self) -> str:
         return repr(asdict(self))
@@ -48,6 +50,9 @@ class Configuration:
 
 
 _config: Configuration | None = None
+# a thread may work under a configuration of its own (a background compile runs under
+# the configuration in force when it was started, whatever init() does meanwhile)
+_local = threading.local()
 
 
 class ConfigurationError(RuntimeError):
@@ -55,16 +60,40 @@ class ConfigurationError(RuntimeError):
 
 
 def get_config() -> Configuration:
+    own = getattr(_local, "config", None)
+    if own is not None:
+        return own
     if _config is None:
         raise ConfigurationError("Please call init first to initialize")
     return _config
+
+
+def config_ready() -> bool:
+    return getattr(_local, "config", None) is not None or _config is not None
+
+
+class thread_config:
+    """Run the current thread under `cfg` (other threads keep the global one)."""
+
+    def __init__(self, cfg: Configuration):
+        self.cfg = cfg
+
+    def __enter__(self):
+        self.saved = getattr(_local, "config", None)
+        _local.config = self.cfg
+        return self.cfg
+
+    def __exit__(self, *exc):
+        _local.config = self.saved
+        return False
 
 
 def init(*, parallel: bool = True, cc: list | None = None, cacheroot: str = ".xgrid",
          comment: bool = False, overstep: Overstep = "none",
          opt_level: int = 2, precision: Precision = "float",
          device: int | None = None, fp_contract: bool = False,
-         graphs: bool | None = None, fuse: bool | None = None, blocked: bool | None = None) -> None:
+         graphs: bool | None = None, fuse: bool | None = None, blocked: bool | None = None,
+         precompile: bool | None = None) -> None:
     """Initialise the global configuration (``xgrid/util/init.py:48-66``).
 
     ``parallel=False`` keeps the reference's rule that implicit (Jacobi)
@@ -86,8 +115,11 @@ def init(*, parallel: bool = True, cc: list | None = None, cacheroot: str = ".xg
         fuse = os.environ.get("XGRID_FUSE", "1") != "0"
     if blocked is None:
         blocked = os.environ.get("XGRID_BLOCKED", "1") != "0"
+    if precompile is None:
+        precompile = os.environ.get("XGRID_PRECOMPILE", "1") != "0"
     _config = Configuration(parallel=parallel, cc=list(cc or ["gcc", "clang"]),
                             cacheroot=cacheroot, comment=comment, overstep=overstep,
                             opt_level=int(opt_level), precision=precision, device=device,
-                            fp_contract=fp_contract, graphs=graphs, fuse=fuse, blocked=blocked)
+                            fp_contract=fp_contract, graphs=graphs, fuse=fuse, blocked=blocked,
+                            precompile=precompile)
     logger.info("initialized with configuration: %s", _config)

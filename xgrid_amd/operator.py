@@ -9,10 +9,18 @@ of the reference's JIT-compiled C (``Generator(self).result``).
 """
 from __future__ import annotations
 
+import dataclasses
 import re
+import threading
 from typing import Any, Callable
 
 from .types import BaseType
+
+# A background compile parses every operator it reaches into a table of its own
+# (this thread-local one), never into Operator._ir: what the first call parses is
+# decided then, under the configuration and globals in force at that time.
+_private = threading.local()
+_ir_lock = threading.RLock()     # a parse parses the operators it calls
 
 CustomTypecheck = Callable[[list], BaseType]
 
@@ -38,12 +46,15 @@ class Operator:
         self.tick = tick
         self.intrinsic = intrinsic
         self._ir = None
+        self._precompiled = None
         self.__doc__ = getattr(func, "__doc__", None)
 
     def __call__(self, *args: Any) -> Any:
         if self.mode == "kernel":
             if self.native is None:
                 from .backend import build_kernel
+                if self._precompiled is not None:
+                    self._precompiled.wait()      # its code objects are then in the cache
                 self.native, self.depth = build_kernel(self)
             from .grid import Grid
             for arg in args:
@@ -56,13 +67,37 @@ class Operator:
 
     @property
     def ir(self):
+        from .frontend import parse
+        table = getattr(_private, "table", None)
+        if table is not None:                    # a background compile: its own parse
+            f = table.get(self)
+            if f is None:
+                f = table[self] = parse(self.func, self.name, self.mode, self.self_type)
+            return f
         if self._ir is None:
-            from .frontend import parse
-            self._ir = parse(self.func, self.name, self.mode, self.self_type)
-            for inc in self._ir.includes:
-                if inc not in self.includes:
-                    self.includes.append(inc)
+            with _ir_lock:
+                if self._ir is None:
+                    f = parse(self.func, self.name, self.mode, self.self_type)
+                    for inc in f.includes:
+                        if inc not in self.includes:
+                            self.includes.append(inc)
+                    self._ir = f
         return self._ir
+
+    def precompile(self, wait: bool = False) -> "Precompile":
+        """Compile this kernel's device code and host stub now, in a background
+        thread, under the configuration in force now (no GPU is touched).  The first
+        call waits for it and then finds both in the on-disk cache; if the call's own
+        parse produces different code (the configuration or a folded global changed
+        in between), it compiles that instead.  Returns the handle (waited on when
+        `wait`)."""
+        if self.mode != "kernel":
+            raise RuntimeError(f"only kernels are compiled; '{self.name}' is a {self.mode}")
+        if self._precompiled is None or (self._precompiled.done() and self._precompiled.error is not None):
+            self._precompiled = Precompile(self)
+        if wait:
+            self._precompiled.wait()
+        return self._precompiled
 
     @property
     def c_name(self) -> str:
@@ -91,10 +126,67 @@ class Operator:
         return f"<xgrid {self.mode} '{self.name}'>"
 
 
+class Precompile:
+    """A kernel's device code and host stub compiled by a background (daemon)
+    thread into the on-disk cache.  Errors are kept, not raised: the first call
+    builds the kernel itself and reports them there."""
+
+    def __init__(self, op: Operator):
+        from .config import get_config
+        self.op = op
+        self.cfg = dataclasses.replace(get_config())
+        self.error = None
+        self.seconds = None
+        self._done = threading.Event()
+        threading.Thread(target=self._run, name=f"xgrid-precompile-{op.name}", daemon=True).start()
+
+    def _run(self):
+        import time
+        from .backend import lower_kernel
+        from .compilers import compile_device, compile_host
+        from .config import thread_config
+        t0 = time.perf_counter()
+        try:
+            _private.table = {}
+            with thread_config(self.cfg):
+                low, _ = lower_kernel(self.op)
+                compile_device(low.device_src, self.cfg.cacheroot, self.cfg.fp_contract)
+                compile_host(low.stub_src, self.cfg.cacheroot, self.cfg.cc, self.cfg.opt_level)
+        except Exception as e:          # reported by the first call's own build
+            self.error = e
+        finally:
+            _private.table = None
+            self.seconds = time.perf_counter() - t0
+            self._done.set()
+
+    def done(self) -> bool:
+        return self._done.is_set()
+
+    def wait(self, timeout: float | None = None) -> bool:
+        return self._done.wait(timeout)
+
+
+def precompile(*ops, wait: bool = False) -> list:
+    """Start background compiles of several kernels; returns their handles."""
+    handles = [op.precompile() for op in ops]
+    if wait:
+        for h in handles:
+            h.wait()
+    return handles
+
+
 def kernel(*, name: str | None = None, includes: list | None = None, tick: bool = True,
            macro: list | None = None):
+    """The kernel decorator.  When the configuration is already initialised (and its
+    `precompile` option is on), the kernel's compile starts in the background at
+    once, so that its first call does not wait for hipRTC (xgrid/util/ffi.py:59-94
+    compiles on the first call)."""
     def aux(func):
-        return Operator(func, "kernel", name, includes, tick=tick, macro=macro)
+        from .config import config_ready, get_config
+        op = Operator(func, "kernel", name, includes, tick=tick, macro=macro)
+        if config_ready() and get_config().precompile:
+            op.precompile()
+        return op
     return aux
 
 
