@@ -10,7 +10,10 @@ in HBM.  Prints one JSON line:
   algorithmic bytes per launch (per iteration: class cells x 8 B x (arrays read
   + 1)) over its average launch duration, measured with HIP events on the
   backend stream over back-to-back launches, against 8 TB/s; `traffic` is the
-  HBM bytes per launch from the committed rocprofv3 counter summary;
+  HBM bytes per launch from the committed rocprofv3 counter summary, with the
+  measured HBM rate and the VALU figure of the committed SQ-counter summary
+  beside it (the blocked kernel keeps its field on chip across iterations, so
+  its algorithmic rate can exceed the HBM peak);
 * cpu_baseline: the oracle's restatement of the reference CPU path (gcc
   -O3 -fopenmp) on a bounded sample of the same workload on the host cores.
 
@@ -144,7 +147,6 @@ def main():
                        f"statements in {launches_per_trip} launches of <= {L.tmax}, {L.core[0]}x{L.core[1]} "
                        f"tiles, halo {L.halo})")
         kernel_id = f"{L.kernel}:{L.core[0]}x{L.core[1]}:h{L.halo}:t{L.tmax}:w{L.threads}:n{n}"
-    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
 
     name = ctypes_name(lib)
     out = {
@@ -163,11 +165,7 @@ def main():
         "config": {"workload": f"lid-driven cavity {n}x{n} FP64, 1+50 Jacobi sweeps/step, dt={args.dt}, "
                                "plain @xgrid.kernel calls",
                    "grid": [n, n], "sweeps_per_step": 51, "device": name},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(kernel_id),
-                     "kernel_id": kernel_id,
-                     "kernel": kernel_name, "bytes_per_launch": bytes_per_launch,
-                     "avg_launch_ms": avg_ms},
+        "roofline": roofline_entry(kernel_id, kernel_name, bytes_per_launch, avg_ms),
         "jacobi_sweep": {"kernel": f"xg_k{jac}_pp", "bytes_per_launch": jac_bytes, "avg_launch_ms": sweep_ms,
                          "GBs": jac_bytes / (sweep_ms * 1e-3) / 1e9,
                          "avg_launch_ms_in_steps": ev_ms, "launches_in_steps": launches},
@@ -391,8 +389,64 @@ def leg_serving(args) -> dict:
                                      workload=f"{len(low)} requests, seed 3, Poisson {args.serve_low_rate}/s")}
 
 
+def leg_compile(args) -> dict:
+    """First-call latency of a kernel the process has not built yet (the 2049^2 FP64
+    cavity under a name and host macro of its own, so that neither the on-disk cache
+    nor the compiler's cache holds it): cold (hipRTC and the host stub compile inside
+    the first call), warm (the same text again: both come from the on-disk cache) and
+    hidden (the compile starts in the background when the kernel is decorated, while
+    the caller allocates and fills its grids, as a script does)."""
+    import uuid
+    import xgrid_amd
+    from xgrid_amd import _lib
+    from xgrid_amd import workloads as W
+    from xgrid_amd.operator import Operator
+    xgrid_amd.init(precision="double", cacheroot=os.path.join(ROOT, ".xgrid"), precompile=False)
+    lib = _lib.load()
+    n = args.n
+    func = W.make_cavity().func
+    cfg = W.cavity_config(n, dt=2e-6)
+    masks = W.cavity_masks(n)
+
+    def setup():
+        grids = [xgrid_amd.Grid((n, n), float) for _ in "bpuv"]
+        for g, c in zip(grids, "bpuv"):
+            g.boundary[...] = masks[c]
+        return grids
+
+    def first_call(op, grids) -> float:
+        t0 = time.perf_counter()
+        op(*grids, cfg)
+        _lib.check(lib.xg_synchronize())
+        return time.perf_counter() - t0
+
+    tag = uuid.uuid4().hex[:12]
+    grids = setup()
+    cold = first_call(Operator(func, "kernel", f"cavity_cold_{tag}", macro=[f"/* {tag} */"]), grids)
+    warm = first_call(Operator(func, "kernel", f"cavity_cold_{tag}", macro=[f"/* {tag} */"]), grids)
+    del grids
+    t0 = time.perf_counter()
+    op = Operator(func, "kernel", f"cavity_hidden_{tag}", macro=[f"/* {tag} hidden */"])
+    pre = op.precompile()                         # what @kernel() does when the configuration is set
+    grids = setup()
+    setup_s = time.perf_counter() - t0
+    hidden = first_call(op, grids)
+    # decorated long before its first call (at import, say): the compile is over by then
+    op2 = Operator(func, "kernel", f"cavity_early_{tag}", macro=[f"/* {tag} early */"])
+    pre2 = op2.precompile(wait=True)
+    early = first_call(op2, grids)
+    return {"workload": f"first call of a new {n}x{n} FP64 cavity kernel (unique source text)",
+            "cold_first_call_s": cold, "warm_first_call_s": warm,
+            "hidden_during_setup": {"first_call_s": hidden, "setup_s_before_call": setup_s,
+                                    "background_compile_s": pre.seconds,
+                                    "compile_error": repr(pre.error) if pre.error else None},
+            "compiled_before_call": {"first_call_s": early, "background_compile_s": pre2.seconds},
+            "note": "each figure includes the call's own device work (one cavity step, plus the blocked loop's "
+                    "validation launch) and the module load"}
+
+
 LEGS = {"diffusion_1d": leg_diffusion_1d, "cavity_fp32": leg_cavity_fp32, "heat_3d_fp32": leg_heat_3d,
-        "batching": leg_batching, "serving": leg_serving}
+        "batching": leg_batching, "serving": leg_serving, "compile": leg_compile}
 
 
 def copy_probe(mib: int, iters: int = 20) -> None:
@@ -404,6 +458,38 @@ def copy_probe(mib: int, iters: int = 20) -> None:
     _lib.check(lib.xg_copy_probe(mib << 20, iters, ctypes.byref(g), ctypes.byref(ms)))
     print(json.dumps({"copy_probe_MiB": mib, "iters": iters, "GBs": g.value, "ms_per_copy": ms.value,
                       "bytes_read_per_copy": mib << 20, "bytes_written_per_copy": mib << 20}), flush=True)
+
+
+def roofline_entry(kernel_id: str, kernel_name: str, bytes_per_launch: float, avg_ms: float) -> dict:
+    """The dominant kernel against the HBM roofline.  `achieved` is the algorithmic
+    (unblocked-sweep) bytes per launch over the launch time; a temporally blocked
+    kernel can exceed the peak on that count, so the measured HBM bytes, their rate
+    and the VALU figure from the committed counter summaries stand beside it."""
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    traffic = measured_traffic(kernel_id)
+    out = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel_id": kernel_id, "kernel": kernel_name,
+           "bytes_per_launch": bytes_per_launch, "avg_launch_ms": avg_ms}
+    if traffic:
+        out["hbm_measured_GBs"] = traffic / (avg_ms * 1e-3) / 1e9
+        out["hbm_measured_frac"] = out["hbm_measured_GBs"] / HBM_PEAK_GBS
+    sq = measured_counters(kernel_id)
+    if sq:
+        out["valu_busy"] = sq.get("valu_busy")
+        out["lds_wait_frac"] = sq.get("lds_wait_frac")
+    if traffic and out["frac"] > 1.0:
+        out["note"] = ("temporally blocked: 'achieved' counts every sweep's algorithmic bytes, while HBM moves "
+                       "'traffic' bytes per launch; the kernel is bound on chip (VALU, LDS), see valu_busy")
+    return out
+
+
+def measured_counters(kernel: str):
+    """SQ-counter figures of `kernel` from the committed rocprofv3 summary, or None."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "sq_counters.json")) as f:
+            return json.load(f).get("kernels", {}).get(kernel)
+    except (OSError, ValueError):
+        return None
 
 
 def measured_traffic(kernel: str):
@@ -429,7 +515,15 @@ def ctypes_name(lib) -> str:
 
 
 def cpu_baseline(n: int, dt: float, seconds: float) -> dict:
-    """The oracle's restated reference CPU path on the same cavity, a bounded sample."""
+    """The oracle's restated reference CPU path on the same workloads, bounded
+    samples: the FP64 cavity (the headline), the 1-D diffusion, the 512^3 FP32
+    heat equation and the FP64 batching mix."""
+    from xgrid_amd.precision import precision_scope
+    with precision_scope("double"):        # the secondary legs may leave FP32 in force
+        return _cpu_baseline(n, dt, seconds)
+
+
+def _cpu_baseline(n: int, dt: float, seconds: float) -> dict:
     import xgrid_amd
     from xgrid_amd import workloads as W
     from oracle.cpu_path import CpuKernel, HostGrid
@@ -477,13 +571,87 @@ def cpu_baseline(n: int, dt: float, seconds: float) -> dict:
     for _ in range(500):
         ckd(ud, 0.3, ddt, dx)
     d_us = (time.perf_counter() - t0) / 500 * 1e6
+    del grids
     return {"value": n * n * steps / el, "unit": "cell-updates/s", "cores": threads, "kind": "port",
             "sample": f"{steps} steps of the {n}x{n} cavity ({el:.1f} s), gcc -O3 -fopenmp, "
                       f"OMP_NUM_THREADS={threads}",
             "single_thread": {"value": n * n * steps1 / el1, "cores": 1,
                               "sample": f"{steps1} steps of the {n}x{n} cavity ({el1:.1f} s), one thread"},
             "diffusion_1d": {"us_per_call": d_us, "cores": threads,
-                             "sample": "500 calls of the 1-D diffusion, 4096 cells FP64"}}
+                             "sample": "500 calls of the 1-D diffusion, 4096 cells FP64"},
+            "heat_3d": cpu_heat_3d(threads, seconds / 3),
+            "batching": cpu_batching(threads, seconds / 2)}
+
+
+def cpu_heat_3d(threads: int, seconds: float, n: int = 512) -> dict:
+    """The restated reference CPU path (true 3-D strides) on the 512^3 FP32 heat
+    equation: a prefix of the 1000 steps, timed per step."""
+    import numpy as np
+    import xgrid_amd
+    from xgrid_amd import workloads as W
+    from xgrid_amd.precision import precision_scope
+    from oracle.cpu_path import CpuKernel, HostGrid
+    with precision_scope("float"):
+        ck = CpuKernel(W.make_heat_3d(), opt_level=3)
+        ck.threads(threads)
+        u0, mask, dx = W.heat_3d_setup(n, np.float32, seed=0)
+        nu = 0.1
+        dt = 0.1 * dx * dx / nu
+        u = HostGrid((n, n, n), float)
+        u.now[...] = u0
+        u.boundary[...] = mask
+        del u0
+        ck(u, nu, dt, dx)                     # JIT compile and first touch of the levels
+        steps = 0
+        t0 = time.perf_counter()
+        while True:
+            ck(u, nu, dt, dx)
+            steps += 1
+            el = time.perf_counter() - t0
+            if el >= seconds or steps >= 20:
+                break
+    return {"s_per_step": el / steps, "cell_updates_per_s": n ** 3 * steps / el, "cores": threads, "kind": "port",
+            "sample": f"{steps} of the 1000 steps of the {n}^3 FP32 heat equation ({el:.1f} s), gcc -O3 -fopenmp; "
+                      f"the whole run extrapolates to {el / steps * 1000:.0f} s"}
+
+
+def cpu_batching(threads: int, seconds: float) -> dict:
+    """CPU solves/s of the batching mix (256 cavity / 2-D diffusion instances, seed
+    1, FP64, a solve = 100 steps of one instance) on the restated reference CPU
+    path: every 8th instance (sorted by kind, then size) for a few steps each,
+    scaled to all 256 instances and 100 steps."""
+    import xgrid_amd
+    from xgrid_amd import workloads as W
+    from oracle.cpu_path import CpuKernel, HostGrid
+    specs = W.batch_specs(256, seed=1)
+    kernels = {"cavity": CpuKernel(W.make_cavity(), opt_level=3), "diffusion": CpuKernel(W.make_diffusion_2d(), opt_level=3)}
+    for ck in kernels.values():
+        ck.threads(threads)
+    order = sorted(specs, key=lambda s: (s.kind, s.n))
+    sample = order[::8]
+    # per-instance seconds per step, first call (first touch) excluded
+    per_step = {}
+    budget = seconds / len(sample)
+    for s in sample:
+        grids, scalars = W.instance_grids(s, HostGrid)
+        ck = kernels[s.kind]
+        ck(*grids, *scalars)
+        k = 0
+        t0 = time.perf_counter()
+        while True:
+            ck(*grids, *scalars)
+            k += 1
+            el = time.perf_counter() - t0
+            if el >= budget or k >= 10:
+                break
+        per_step[s.rid] = el / k
+    # each sampled instance stands for the 8 around it in the sorted order (same kind, near size)
+    step_s = sum(per_step[s.rid] * len(order[i * 8:(i + 1) * 8]) for i, s in enumerate(sample))
+    K = 100
+    return {"solves_per_s": len(specs) / (step_s * K), "s_per_step_all_instances": step_s, "cores": threads,
+            "kind": "port", "steps_per_solve": K,
+            "sample": f"{len(sample)} of the {len(specs)} instances (every 8th by kind and size), up to 10 steps "
+                      f"each; each stands for the 8 instances around it; FP64, gcc -O3 -fopenmp"}
 
 
 if __name__ == "__main__":

@@ -147,8 +147,9 @@ class CSource:
         name = t.name
         if name in self.types:
             return
-        self.types[name] = ""
+        # field types first: the emitted type definitions keep this insertion order
         fields_ = "".join(f"  {self.type_name(ft)} {fn};\n" for fn, ft in t.elements)
+        self.types[name] = ""
         self.decls.append(f"struct {name};")
         self.types[name] = f"struct {name} {{\n{fields_}}};\n"
 
@@ -165,8 +166,8 @@ class CSource:
         name = f"__Grid{t.dimension}d_{_abbr(t.element)}"
         if name in self.types:
             return
+        el = self.type_name(t.element)      # a structure element is defined before the grid
         self.types[name] = ""
-        el = self.type_name(t.element)
         D = t.dimension
         self.decls.append(f"struct {name};")
         s = [f"struct {name} {{", "  int32_t time;", f"  int32_t shape[{D}];", f"  {el}** data;",
@@ -429,7 +430,8 @@ class CpuKernel:
                 keep.append(mask)
                 ptrs = (el * len(a._data))(*[lvl.ctypes.data_as(el) for lvl in a._data])
                 keep.append(ptrs)
-                cargs.append(ct(len(a._data), (ctypes.c_int32 * t.dimension)(*a.shape), ptrs,
+                shape_t = dict(ct._fields_)["shape"]          # the struct's own array type
+                cargs.append(ct(len(a._data), shape_t(*a.shape), ptrs,
                                 mask.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))))
             elif isinstance(t, Structure):
                 cargs.append(t.to_c(a))

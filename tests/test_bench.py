@@ -46,3 +46,15 @@ def test_cpu_baseline_reports_cores_and_sample():
     assert r["kind"] == "port" and r["unit"] == "cell-updates/s" and r["value"] > 0
     assert r["cores"] >= 1 and "65x65" in r["sample"]
     assert r["single_thread"]["cores"] == 1 and r["single_thread"]["value"] > 0
+
+
+def test_roofline_entry_reports_measured_traffic_and_valu_beside_algorithmic_rate():
+    """A blocked kernel's algorithmic rate may exceed the HBM peak; the entry then
+    carries the measured HBM rate (below the peak) and the VALU figure."""
+    import bench
+    kid = "xg_tb0:48x40:h12:t10:w1024:n2049"
+    r = bench.roofline_entry(kid, "xg_tb0", 1006796960.0, 0.111)
+    assert r["bound"] == "hbm" and r["unit"] == "GB/s" and r["peak"] == 8000.0
+    assert abs(r["frac"] - r["achieved"] / 8000.0) < 1e-12 and r["frac"] > 1.0
+    assert 0 < r["hbm_measured_frac"] < 0.5 and r["traffic"] < 1006796960.0
+    assert 0 < r["valu_busy"] < 1 and "note" in r

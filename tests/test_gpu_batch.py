@@ -55,10 +55,11 @@ def test_batched_cavities_equal_alone(double, graphs, max_blocks):
     batch, members = _run_cavities(double, sizes, steps, graphs, max_blocks, nus)
     for gb, ga, cfg in members:
         _assert_same(gb, ga)
-    gb, ga, cfg = members[0]
-    ref = R.cavity(33, steps, cfg.rho, cfg.nu, cfg.dt, cfg.dx, cfg.dy)
-    for g, r in zip(gb, ref):
-        assert np.array_equal(g.now, r)
+    # every member also equals the numpy restatement of the reference's cavity
+    for (gb, ga, cfg), n in zip(members, sizes):
+        ref = R.cavity(n, steps, cfg.rho, cfg.nu, cfg.dt, cfg.dx, cfg.dy)
+        for g, r in zip(gb, ref):
+            assert np.array_equal(g.now, r), (n, "batched member vs restatement")
     c = batch.counters()
     assert c["steps"] == steps and c["validations"] >= 1
     if graphs:
@@ -69,10 +70,25 @@ def test_batched_cavities_equal_alone(double, graphs, max_blocks):
 
 
 def test_batched_cavities_fp32_equal_alone(single):
-    batch, members = _run_cavities(single, [48, 80, 33], 9)
-    for gb, ga, cfg in members:
+    from oracle.cpu_path import CpuKernel, HostGrid
+    from xgrid_amd import workloads as W
+    sizes, steps = [48, 80, 33], 9
+    batch, members = _run_cavities(single, sizes, steps)
+    ck = CpuKernel(W.make_cavity())
+    for (gb, ga, cfg), n in zip(members, sizes):
         assert gb[1].now.dtype == np.float32
         _assert_same(gb, ga)
+        # and the FP32 oracle (the restated reference CPU path in FP32)
+        masks = W.cavity_masks(n)
+        hs = []
+        for c in "bpuv":
+            h = HostGrid((n, n), float)
+            h.boundary[...] = masks[c]
+            hs.append(h)
+        for _ in range(steps):
+            ck(*hs, cfg)
+        for g, h in zip(gb, hs):
+            assert np.array_equal(g.now.view(np.uint32), h.now.view(np.uint32)), (n, "batched member vs oracle")
     batch.close()
 
 

@@ -11,6 +11,7 @@ import pytest
 import xgrid_amd as xgrid
 from oracle.cpu_path import CpuKernel, HostGrid
 from tests.helpers import assert_bitexact, grid_pair, run_pair
+from xgrid_amd import workloads as W
 
 pytestmark = pytest.mark.gpu
 
@@ -747,3 +748,97 @@ def test_bool_grid_statements(double):
     assert df.now.dtype == np.bool_
     assert_bitexact(du, hu)
     assert np.array_equal(df.now[:, :-1], hf.now[:, :-1])
+
+
+def test_struct_element_grid_matches_oracle(double):
+    """A grid of structures (xgrid/xgrid/__init__.py:10-18: a structured numpy dtype
+    in C layout) on the device: fields of neighbouring cells read, a constructor
+    stored, walls reset; bit for bit against the oracle and its numpy restatement."""
+    from tests.helpers import Cplx, struct_kernel, struct_restated, struct_setup
+    k = struct_kernel()
+    shape = (70, 131)
+    re, im, tag = struct_setup(shape, 45)
+    d, h = grid_pair(xgrid, shape, Cplx, W.box_walls(shape))
+    for g in (d, h):
+        g.now["re"], g.now["im"], g.now["tag"] = re, im, tag
+    run_pair(k, (d, 0.75), (h, 0.75), steps=4)
+    assert d.now.dtype == h.now.dtype and d.now.dtype.itemsize == 24
+    assert_bitexact(d, h)
+    wre, wim, wtag = struct_restated(re, im, tag, 0.75, 4)
+    assert np.array_equal(d.now["re"], wre) and np.array_equal(d.now["im"], wim)
+    assert np.array_equal(d.now["tag"], wtag)
+
+
+def _ulps(a, b):
+    """Distance in units in the last place between two float64 arrays (finite values)."""
+    ia, ib = a.view(np.int64).astype(object), b.view(np.int64).astype(object)
+    ia = [x if x >= 0 else -(x & 0x7fffffffffffffff) for x in ia]
+    ib = [x if x >= 0 else -(x & 0x7fffffffffffffff) for x in ib]
+    return max(abs(x - y) for x, y in zip(ia, ib))
+
+
+def test_math_externals_inside_stencil_statements(double):
+    """@xgrid.external <math.h> functions called inside stencil statements run on
+    the device (the reference's generated C calls them inside its loop nests,
+    xgrid/lang/generator.py:204-212).  Correctly rounded ones (sqrt, fabs, floor,
+    fmin, fmax, fmod, copysign, fma) give the oracle's bits; the transcendental
+    ones come from the device's math library instead of glibc and are held to
+    4 ulp of the oracle."""
+    from xgrid_amd.types import Floating
+    dbl = lambda args: Floating(8)      # noqa: E731
+
+    @xgrid.external(name="sqrt", includes=["math.h"], typecheck_override=dbl)
+    def c_sqrt(x: float) -> float: ...
+
+    @xgrid.external(name="fabs", includes=["math.h"], typecheck_override=dbl)
+    def c_fabs(x: float) -> float: ...
+
+    @xgrid.external(name="floor", includes=["math.h"], typecheck_override=dbl)
+    def c_floor(x: float) -> float: ...
+
+    @xgrid.external(name="fmin", includes=["math.h"], typecheck_override=dbl)
+    def c_fmin(x: float, y: float) -> float: ...
+
+    @xgrid.external(name="fmod", includes=["math.h"], typecheck_override=dbl)
+    def c_fmod(x: float, y: float) -> float: ...
+
+    @xgrid.external(name="fma", includes=["math.h"], typecheck_override=dbl)
+    def c_fma(x: float, y: float, z: float) -> float: ...
+
+    @xgrid.external(name="sin", includes=["math.h"], typecheck_override=dbl)
+    def c_sin(x: float) -> float: ...
+
+    @xgrid.external(name="exp", includes=["math.h"], typecheck_override=dbl)
+    def c_exp(x: float) -> float: ...
+
+    @xgrid.external(name="atan2", includes=["math.h"], typecheck_override=dbl)
+    def c_atan2(x: float, y: float) -> float: ...
+
+    @xgrid.kernel()
+    def exact(u: f2, v: f2) -> None:
+        u[0, 0] = c_sqrt(c_fabs(v[0, 0] - v[1, 0])) + c_floor(v[0, 1] * 7.5) - c_fmin(v[0, -1], 0.5) + \
+            c_fmod(v[-1, 0] * 10.0, 0.3) + c_fma(v[0, 0], v[0, 1], -v[1, 0])
+
+    # one function per output grid, so that a last-place difference is not magnified
+    # by cancellation in a sum
+    @xgrid.kernel()
+    def transcendental(s: f2, e: f2, t: f2, v: f2) -> None:
+        s[0, 0] = c_sin(v[0, 1] * 3.0)
+        e[0, 0] = c_exp(v[0, 0])
+        t[0, 0] = c_atan2(v[1, 0] - 0.5, v[-1, 0] + 0.1)
+
+    rng = np.random.default_rng(46)
+    v0 = rng.random((40, 70)) * 2.0 - 0.5
+    walls = W.box_walls((40, 70))
+    du, hu = grid_pair(xgrid, (40, 70), float, walls)
+    dv, hv = grid_pair(xgrid, (40, 70), float, None, v0)
+    run_pair(exact, (du, dv), (hu, hv), steps=1)
+    assert_bitexact(du, hu)
+    outs = [grid_pair(xgrid, (40, 70), float, walls) for _ in range(3)]
+    dv, hv = grid_pair(xgrid, (40, 70), float, None, v0)
+    run_pair(transcendental, [d for d, _ in outs] + [dv], [h for _, h in outs] + [hv], steps=1)
+    for d, h in outs:
+        a, b = d.now[1:-1, 1:-1], h.now[1:-1, 1:-1]
+        assert np.isfinite(a).all() and _ulps(a.ravel(), b.ravel()) <= 4
+    for k in (exact, transcendental):
+        assert k.native.lowered.graphs_ok                   # pure device functions: graphs stay on

@@ -19,6 +19,7 @@ import xgrid_amd as xgrid
 from xgrid_amd import workloads as W
 from oracle import restate as R
 from oracle.cpu_path import CpuKernel, CSource, HostGrid
+from tests.helpers import Cplx, struct_kernel, struct_restated, struct_setup
 
 
 def _host(shape, dtype, init=None, mask=None):
@@ -262,3 +263,125 @@ def test_planted_mistakes_fail_the_checks(double, monkeypatch):
         "_value[(_d0) * p.shape[1] + _d1] = ", "*__Grid2d_d_at(p, _d0 + 0, _d1 + 0, 0) = "))
     # loads of the previous level read the current one
     assert not run_with(lambda self: orig(self).replace("_d1 + 1, 1))", "_d1 + 1, 0))"))
+
+
+# ----------------------------------------------------------- DSL features outside the workloads
+def _c_div(a, d):
+    """C integer division and remainder (truncation toward zero), from numpy floor ops."""
+    q = np.abs(a) // np.abs(d) * np.sign(a) * np.sign(d)
+    return q, a - q * d
+
+
+@pytest.mark.parametrize("mode", ["limit", "wrap"])
+def test_overstep_2d_matches_padding_and_roll(mode):
+    """generator.py:172-177: `limit` clamps each coordinate (numpy pad mode "edge"),
+    `wrap` takes it modulo the extent (numpy roll); a square grid, where the
+    reference's index formula is row-major."""
+    xgrid.init(precision="double", overstep=mode)
+
+    @xgrid.kernel()
+    def edge(u: xgrid.grid[float, 2]) -> None:
+        u[0, 0] = (u[0, 1] + u[2, 0]) * 0.5 - u[-1, -1] * 0.25 + u[0, -2]
+
+    rng = np.random.default_rng(41)
+    u0 = rng.random((23, 23))
+    g = _host((23, 23), float, u0)
+    CpuKernel(edge, overstep=mode)(g)
+    if mode == "limit":
+        P = np.pad(u0, 2, mode="edge")
+        at = lambda o0, o1: P[2 + o0:2 + o0 + 23, 2 + o1:2 + o1 + 23]    # noqa: E731
+    else:
+        at = lambda o0, o1: np.roll(u0, (-o0, -o1), axis=(0, 1))         # noqa: E731
+    want = (at(0, 1) + at(2, 0)) * 0.5 - at(-1, -1) * 0.25 + at(0, -2)
+    assert np.array_equal(g.now, want)
+
+
+def test_three_time_levels_leapfrog():
+    """parser.py:512-525 and xgrid/xgrid/__init__.py:43-54: loads at times -1 and -2
+    read the two previous calls' levels (the ring rotates the oldest level to the
+    front, and new levels start at zero), checked against a numpy leapfrog."""
+    xgrid.init(precision="double")
+
+    @xgrid.kernel()
+    def wave(u: xgrid.grid[float, 1], c: float) -> None:
+        u[0] = 2.0 * u[0][-1] - u[0][-2] + c * (u[1][-1] - 2.0 * u[0][-1] + u[-1][-1])
+        with xgrid.boundary(1):
+            u[0] = 0.0
+
+    n = 50
+    rng = np.random.default_rng(42)
+    u0 = rng.random(n)
+    mask = np.zeros(n, np.int32)
+    mask[0] = mask[-1] = 1
+    g = _host((n,), float, u0, mask)
+    ck = CpuKernel(wave)
+    assert ck.depth == 3
+    prev, prev2 = u0.copy(), np.zeros(n)
+    for _ in range(7):
+        ck(g, 0.3)
+        new = np.zeros(n)
+        new[1:-1] = 2.0 * prev[1:-1] - prev2[1:-1] + 0.3 * (prev[2:] - 2.0 * prev[1:-1] + prev[:-2])
+        prev2, prev = prev, new
+        assert np.array_equal(g.now, new)
+        assert np.array_equal(g._data[1], prev2)
+
+
+def test_shifted_two_phase_store():
+    """generator.py:312-352: an implicit statement with a store offset computes every
+    class cell into a temporary first, then stores it at the shifted cell: here
+    p[i+1] = p[i-1] for the class-0 cells 1..n-2, all reads before any write."""
+    xgrid.init(precision="double")
+
+    @xgrid.kernel()
+    def shift(p: xgrid.grid[float, 1]) -> None:
+        p[1] = p[-1][0] * 1.5
+
+    n = 31
+    rng = np.random.default_rng(43)
+    p0 = rng.random(n)
+    mask = np.zeros(n, np.int32)
+    mask[0] = mask[-1] = 1
+    g = _host((n,), float, p0, mask)
+    ck = CpuKernel(shift)
+    want = p0.copy()
+    for _ in range(3):
+        ck(g, )
+        nxt = want.copy()
+        nxt[2:] = want[:-2] * 1.5
+        want = nxt
+        assert np.array_equal(g.now, want)
+
+
+def test_integer_division_and_remainder_truncate():
+    """C integer '/' and '%' (generator.py:377-385 emits them as written) truncate
+    toward zero, unlike Python's floor division, on negative operands too."""
+    xgrid.init(precision="double")
+    i1 = xgrid.grid[int, 1]
+
+    @xgrid.kernel()
+    def idiv(q: i1, r: i1, a: i1, d: int) -> None:
+        q[0] = a[0] / d
+        r[0] = a[0] % d
+
+    a0 = np.arange(-40, 41, dtype=np.int32) * 7 + 3
+    for d in (3, -5, 7, -1):
+        q, r, a = _host((81,), int), _host((81,), int), _host((81,), int, a0)
+        CpuKernel(idiv)(q, r, a, d)
+        wq, wr = _c_div(a0.astype(np.int64), d)
+        assert np.array_equal(q.now, wq) and np.array_equal(r.now, wr), d
+        assert (np.abs(r.now) < abs(d)).all() and ((r.now == 0) | (np.sign(r.now) == np.sign(a0))).all()
+
+
+def test_struct_grid_matches_restatement():
+    xgrid.init(precision="double")
+    shape = (12, 17)
+    re, im, tag = struct_setup(shape, 44)
+    g = _host(shape, Cplx, mask=W.box_walls(shape))
+    assert g.now.dtype.itemsize == 24 and g.now.dtype.fields["tag"][1] == 16     # C layout, aligned
+    g.now["re"], g.now["im"], g.now["tag"] = re, im, tag
+    ck = CpuKernel(struct_kernel())
+    for _ in range(4):
+        ck(g, 0.75)
+    wre, wim, wtag = struct_restated(re, im, tag, 0.75, 4)
+    assert np.array_equal(g.now["re"], wre) and np.array_equal(g.now["im"], wim)
+    assert np.array_equal(g.now["tag"], wtag)
