@@ -57,7 +57,9 @@ def main():
                     help="batching leg: time the batched steps only (no plain comparison), e.g. under a profiler")
     ap.add_argument("--batch-max-blocks", type=int, default=0,
                     help="persistent grid cap of batched launches (0: one workgroup per tile)")
-    ap.add_argument("--serve-requests", type=int, default=48, help="requests of the serving trace (400 = full)")
+    ap.add_argument("--serve-requests", type=int, default=48, help="requests of the serving trace's prefix")
+    ap.add_argument("--serve-full-requests", type=int, default=400,
+                    help="requests of the whole serving trace, served after the prefix (0: skip)")
     ap.add_argument("--tune-limit", type=int, default=24,
                     help="instances the autotuner times per kernel in the batching/serving legs (0: no tuning)")
     ap.add_argument("--serve-budget-gib", type=float, default=16.0)
@@ -283,6 +285,7 @@ def leg_batching(args) -> dict:
     seed 1): K steps of all of them batched against the same K steps run instance by
     instance with plain kernel calls; results must be identical."""
     import numpy as np
+    import xgrid_amd
     from xgrid_amd import workloads as W
     from xgrid_amd.multi import MultiSolver
     from xgrid_amd import _lib
@@ -310,6 +313,7 @@ def leg_batching(args) -> dict:
         ms.step(K)
         _lib.check(lib.xg_synchronize())
         t_batch = time.perf_counter() - t0
+        print(f"# batching {prec}: batched steps done", file=sys.stderr, flush=True)
         key = "f64" if prec == "double" else "f32"
         if args.batch_no_plain:
             out[key] = {"batched_s": t_batch, "ms_per_batched_step": t_batch / K * 1e3,
@@ -318,23 +322,43 @@ def leg_batching(args) -> dict:
             ms.close()
             continue
         plain_before = {k: op.native.counters() for k, op in ms.ops.items()}
-        plain = []
-        for s in specs:
-            grids, scalars = ms.grids_for(s)
-            for _ in range(warm):
-                ms.plain_call(s, grids, scalars)
-            plain.append((grids, scalars))
-        _lib.check(lib.xg_synchronize())
-        t0 = time.perf_counter()
-        for s, (grids, scalars) in zip(specs, plain):
-            for _ in range(K):
-                ms.plain_call(s, grids, scalars)
-        _lib.check(lib.xg_synchronize())
-        t_plain = time.perf_counter() - t0
-        same = all(np.array_equal(a.now, b.now) for h, (grids, _) in zip(handles, plain)
-                   for a, b in zip(ms.grids(h), grids))
+        # The plain side is timed twice, with graph replay on (the default) and off, and
+        # the faster is the baseline: going from instance to instance, each instance's
+        # graphs are captured again after its predecessors evicted them, which costs the
+        # replays' gain when the device is the bottleneck.
+        # With graph replay switched off process-wide (XGRID_GRAPHS=0, as for a profiled
+        # run), only the graphs-off pass runs.
+        plain_times = {}
+        for graphs in ((True, False) if xgrid_amd.get_config().graphs else (False,)):
+            for op in ms.ops.values():
+                if op.native is not None:
+                    op.native.set_options(graphs=graphs)
+            plain = []
+            for s in specs:
+                grids, scalars = ms.grids_for(s)
+                for _ in range(warm):
+                    ms.plain_call(s, grids, scalars)
+                plain.append((grids, scalars))
+            _lib.check(lib.xg_synchronize())
+            t0 = time.perf_counter()
+            for s, (grids, scalars) in zip(specs, plain):
+                for _ in range(K):
+                    ms.plain_call(s, grids, scalars)
+            _lib.check(lib.xg_synchronize())
+            plain_times["graphs_on" if graphs else "graphs_off"] = time.perf_counter() - t0
+            print(f"# batching {prec}: plain calls done (graphs {'on' if graphs else 'off'})",
+                  file=sys.stderr, flush=True)
+            same = all(np.array_equal(a.now, b.now) for h, (grids, _) in zip(handles, plain)
+                       for a, b in zip(ms.grids(h), grids))
+            if not same:
+                break
+            del plain
+        for op in ms.ops.values():
+            if op.native is not None:
+                op.native.set_options()
+        t_plain = min(plain_times.values())
         out[key] = {
-            "batched_s": t_batch, "plain_s": t_plain, "speedup": t_plain / t_batch,
+            "batched_s": t_batch, "plain_s": t_plain, "plain_s_by_graphs": plain_times, "speedup": t_plain / t_batch,
             "solves_per_s_batched": len(specs) / t_batch, "solves_per_s_plain": len(specs) / t_plain,
             "cell_updates_per_s_batched": cells * K / t_batch, "ms_per_batched_step": t_batch / K * 1e3,
             "identical_to_plain": bool(same), "counters": ms.counters(),
@@ -343,8 +367,8 @@ def leg_batching(args) -> dict:
             "plain_counters": {k: {c: v - plain_before[k][c] for c, v in op.native.counters().items()}
                                for k, op in ms.ops.items()}}
         ms.close()
-        del plain
-    out["note"] = "a solve = K steps of one instance; plain = each instance's kernel called K times, one instance after another"
+    out["note"] = ("a solve = K steps of one instance; plain = each instance's kernel called K times, one instance "
+                   "after another, the faster of graph replay on and off")
     return out
 
 
@@ -367,11 +391,23 @@ def leg_serving(args) -> dict:
                 print(f"# serving {label}: {n} of {len(trace)} requests done", file=sys.stderr, flush=True)
         return on_done
 
-    got, summ = srv.run(trace, on_done=progress("batched"))
-    summ.update(peak_resident_requests=srv.peak_active, peak_resident_bytes=srv.peak_used, ticks=srv.ticks)
-    ref, summ_plain = serve_plain(trace, "float", solver=srv.ms, on_done=progress("plain FIFO"))
-    same = all(np.array_equal(got[r].fields[c], ref[r].fields[c]) for r in got for c in "bpuv")
-    del got, ref
+    def served_and_plain(tr, label):
+        got, summ = srv.run(tr, on_done=progress(f"batched{label}"))
+        summ.update(peak_resident_requests=srv.peak_active, peak_resident_bytes=srv.peak_used, ticks=srv.ticks)
+        ref, summ_plain = serve_plain(tr, "float", solver=srv.ms, on_done=progress(f"plain FIFO{label}"))
+        same = all(np.array_equal(got[r].fields[c], ref[r].fields[c]) for r in got for c in "bpuv")
+        return summ, summ_plain, same
+
+    summ, summ_plain, same = served_and_plain(trace, "")
+    full = None
+    if args.serve_full_requests > len(trace):
+        # the whole trace the prefix is taken from, as a second entry of the leg
+        tr_full = W.serving_specs(args.serve_full_requests, seed=2, rate=200.0)
+        f_summ, f_plain, f_same = served_and_plain(tr_full, " (full trace)")
+        full = {"requests": len(tr_full), "cell_steps": sum(s.n * s.n * s.steps for s in tr_full),
+                "batched": f_summ, "plain_fifo": f_plain,
+                "speedup_solves_per_s": f_summ["solves_per_s"] / f_plain["solves_per_s"],
+                "identical_to_plain": bool(f_same)}
     low = W.serving_specs(args.serve_low_requests, seed=3, rate=args.serve_low_rate)
     got_low, summ_low = srv.run(low)
     summ_low.update(peak_resident_requests=srv.peak_active)
@@ -386,7 +422,8 @@ def leg_serving(args) -> dict:
             "identical_to_plain": bool(same), "tile_shapes": {k: list(v) if v else None for k, v in srv.ms.shapes.items()},
             "autotune": {k: {"timings_ms_per_step": v["timings"], "cached": v["cached"]} for k, v in tuning.items()},
             "below_saturation": dict(summ_low, rate_per_s=args.serve_low_rate,
-                                     workload=f"{len(low)} requests, seed 3, Poisson {args.serve_low_rate}/s")}
+                                     workload=f"{len(low)} requests, seed 3, Poisson {args.serve_low_rate}/s"),
+            "full_trace": full}
 
 
 def leg_compile(args) -> dict:

@@ -207,11 +207,18 @@ def test_staged_3d_plan(double, monkeypatch):
     from xgrid_amd.backend import lower_kernel
     f3 = xgrid.grid[float, 3]
 
-    src = lower_kernel(W.make_heat_3d())[0].device_src
+    low = lower_kernel(W.make_heat_3d())[0]
+    src = low.device_src
     s0 = _stmt_struct(src, 0)
-    assert "ST3 = true" in s0 and "ST3_LDS = 12288" in s0           # 4 planes x 6 rows x 64 columns x 8 B
+    assert "ST3 = true" in s0 and "ST3_LDS = 24576" in s0           # 4 planes x 6 rows x 128 columns x 8 B
+    assert "CPL = 2" in s0 and low.stmts[0].tile3_cols == 128        # two columns per lane, 128-column tiles
     assert "xg::ld<" not in s0[s0.index("value_st3"):]              # every load of the 7-point update staged
-    assert "amdgpu_waves_per_eu(8)))" in src[src.index("xg_k0(") - 80:src.index("xg_k0(")]
+    assert "amdgpu_waves_per_eu(6)))" in src[src.index("xg_k0(") - 80:src.index("xg_k0(")]
+    assert "CPL = 1" in _stmt_struct(src, 1) and low.stmts[1].tile3_cols == 64     # the walls: plain loads
+    monkeypatch.setenv("XGRID_STAGE3_COLS", "1")                      # one column per lane, as before
+    low1 = lower_kernel(W.make_heat_3d())[0]
+    assert "ST3_LDS = 12288" in _stmt_struct(low1.device_src, 0) and low1.stmts[0].tile3_cols == 64
+    monkeypatch.delenv("XGRID_STAGE3_COLS")
 
     @xgrid.kernel()
     def diag(u: f3, v: f3) -> None:
@@ -233,25 +240,93 @@ def test_staged_3d_plan(double, monkeypatch):
 
 
 def _dpp_in_masked_regions(asm: str) -> dict:
-    """Per kernel of an AMD assembly listing: (DPP moves, DPP moves between an
-    s_and_saveexec and the s_or_b64 exec that ends its region)."""
+    """Per kernel of an AMD assembly listing: (DPP moves, DPP moves executed
+    inside an exec-masked region).  The kernel is split into basic blocks
+    (labels, and the ends of branches); the nesting depth of exec-masked regions
+    (+1 at s_and_saveexec, -1 at the s_or_b64 exec that ends a region) flows
+    along the branch and fall-through edges from depth 0 at entry, the largest
+    depth winning where paths join.  Following edges rather than layout order
+    matters: the compiler lays a region's blocks out before or after the block
+    that opens it."""
     import re
-    cur, depth, out = None, 0, {}
+    kernels, cur = {}, None
     for line in asm.splitlines():
         m = re.match(r"^([A-Za-z_]\w*):", line)
         if m:
-            cur, depth = m.group(1), 0
+            cur = m.group(1)
+            kernels[cur] = []
             continue
-        t = line.strip()
         if cur is None:
             continue
-        if t.startswith("s_and_saveexec"):
-            depth += 1
-        elif t.startswith("s_or_b64 exec, exec,"):
-            depth = max(0, depth - 1)
-        if "_dpp" in t:
-            n, bad = out.get(cur, (0, 0))
-            out[cur] = (n + 1, bad + (depth > 0))
+        t = line.split(";")[0].strip()
+        if t.startswith(".Lfunc_end"):
+            cur = None
+            continue
+        if t:
+            kernels[cur].append(t)
+    out = {}
+    for name, lines in kernels.items():
+        blocks, labels, b = [], {}, []
+        for t in lines:
+            lm = re.match(r"^(\.LBB\w+):", t)
+            if lm:
+                if b:
+                    blocks.append(b)
+                b = []
+                labels[lm.group(1)] = len(blocks)
+                continue
+            if t.startswith("."):
+                continue
+            b.append(t)
+            if t.startswith(("s_branch", "s_cbranch", "s_endpgm", "s_setpc")):
+                blocks.append(b)
+                b = []
+        if b:
+            blocks.append(b)
+        # labels map to the index of the block that starts after them
+        succ = []
+        for i, blk in enumerate(blocks):
+            last = blk[-1] if blk else ""
+            tgt = last.split()[1] if last.startswith(("s_branch", "s_cbranch")) and len(last.split()) > 1 else None
+            nxt = [i + 1] if i + 1 < len(blocks) else []
+            if last.startswith("s_branch"):
+                succ.append([labels[tgt]] if tgt in labels else [])
+            elif last.startswith("s_cbranch"):
+                succ.append(([labels[tgt]] if tgt in labels else []) + nxt)
+            elif last.startswith(("s_endpgm", "s_setpc")):
+                succ.append([])
+            else:
+                succ.append(nxt)
+        depth_in = [None] * len(blocks)
+        if not blocks:
+            continue
+        depth_in[0] = 0
+        work = [0]
+        while work:
+            i = work.pop()
+            d = depth_in[i]
+            for t in blocks[i]:
+                if t.startswith("s_and_saveexec"):
+                    d += 1
+                elif t.startswith("s_or_b64 exec, exec,"):
+                    d = max(0, d - 1)
+            for j in succ[i]:
+                if depth_in[j] is None or d > depth_in[j]:
+                    depth_in[j] = min(d, 64)
+                    work.append(j)
+        n = bad = 0
+        for i, blk in enumerate(blocks):
+            d = depth_in[i] if depth_in[i] is not None else 0
+            for t in blk:
+                if t.startswith("s_and_saveexec"):
+                    d += 1
+                elif t.startswith("s_or_b64 exec, exec,"):
+                    d = max(0, d - 1)
+                if "_dpp" in t:
+                    n += 1
+                    bad += d > 0
+        if n:
+            out[name] = (n, bad)
     return out
 
 
@@ -260,10 +335,9 @@ def test_staged_lane_shifts_run_on_every_lane(double):
     reads 0 from the disabled ones.  In the generated source every shift is a
     declaration of its own, never an operand of a conditional (which would be
     evaluated only where the condition selects it).  In the compiled gfx950 code
-    (same options as at run time) no DPP move of a plain statement kernel sits
-    in an exec-masked region.  Batched kernels are left out of the second check:
-    their list-domain loop is laid out between a scalar branch and the staged
-    code, which a scan in layout order cannot tell apart from nesting."""
+    (same options as at run time) no DPP move, in plain or batched kernels, is
+    reached inside an exec-masked region.  The check itself is pinned by a
+    planted shift inside the edge lane's conditional, which it must flag."""
     import re
     import shutil
     import subprocess
@@ -285,7 +359,7 @@ def test_staged_lane_shifts_run_on_every_lane(double):
         src = lower_kernel(op)[0].device_src
         shifts = [ln.strip() for ln in src.splitlines() if "lane_shift<" in ln]
         assert shifts, "no staged statement"
-        assert all(re.fullmatch(r"const \w+ [ab] = xg::lane_shift<-?1>\(c\);", ln) for ln in shifts), shifts
+        assert all(re.fullmatch(r"const \w+ [ab] = xg::lane_shift<-?1>\(c[fl]\);", ln) for ln in shifts), shifts
         with tempfile.TemporaryDirectory() as d:
             p = os.path.join(d, "k.hip")
             with open(p, "w") as f:
@@ -295,9 +369,25 @@ def test_staged_lane_shifts_run_on_every_lane(double):
                                capture_output=True, text=True, timeout=300)
             assert r.returncode == 0, r.stderr[-2000:]
             asm = open(os.path.join(d, "k.s")).read()
-        found = {k: v for k, v in _dpp_in_masked_regions(asm).items() if not k.endswith("_b")}
+        found = _dpp_in_masked_regions(asm)
         assert found, "no staged kernel with lane shifts"
         assert all(bad == 0 for _, bad in found.values()), found
+        if op is not mixed:
+            continue
+        # the planted mistake: a shift evaluated only where the conditional selects it
+        planted = re.sub(r"const (\w+) a = (xg::lane_shift<1>\(cf\));\n(\s*)(st\.xp\d+) = (st\.lane == 63 \? [^:]+ : )a;",
+                         r"\3\4 = \5\2;", src, count=1)
+        assert planted != src
+        with tempfile.TemporaryDirectory() as d:
+            p = os.path.join(d, "k.hip")
+            with open(p, "w") as f:
+                f.write(planted)
+            r = subprocess.run([hipcc, "-x", "hip", "--cuda-device-only", "-S", *compilers.device_options(),
+                                "-I", compilers.CSRC, p, "-o", os.path.join(d, "k.s")],
+                               capture_output=True, text=True, timeout=300)
+            assert r.returncode == 0, r.stderr[-2000:]
+            bad = _dpp_in_masked_regions(open(os.path.join(d, "k.s")).read())
+        assert any(b > 0 for _, b in bad.values()), bad
 
 
 def test_statement_pairs_marked(double):

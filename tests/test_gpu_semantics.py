@@ -4,6 +4,7 @@ and the reference's known answers (test.py)."""
 import ctypes
 import dataclasses
 import random
+import re
 
 import numpy as np
 import pytest
@@ -181,7 +182,9 @@ def test_3d_staged_neighbours_of_other_planes(double):
     from xgrid_amd import workloads as W
     from xgrid_amd.backend import lower_kernel
     src = lower_kernel(k)[0].device_src
-    assert "ST3 = true" in src and "ST3_LDS = 24576" in src       # two plane offsets staged in LDS
+    cpl = int(re.search(r"static constexpr int CPL = (\d)", src).group(1))
+    # two plane offsets staged in LDS: 4 planes x 6 rows x (64 x CPL) columns of 8 B each
+    assert "ST3 = true" in src and f"ST3_LDS = {2 * 4 * 6 * 64 * cpl * 8}" in src
     rng = np.random.default_rng(12)
     shape = (14, 19, 140)
     du, hu = grid_pair(xgrid, shape, float, W.box_walls(shape), rng.random(shape))
@@ -292,12 +295,21 @@ def test_list_and_masked_classes(double):
     assert_bitexact(dv, hv)
 
 
-@pytest.mark.parametrize("shape", [(9, 11, 150), (6, 7, 64), (5, 40, 200)])
+@pytest.mark.parametrize("shape", [(9, 11, 150), (6, 7, 64), (5, 40, 200), (7, 9, 259), (6, 10, 300)])
 def test_3d_classes_at_any_column(double, shape):
     """3-D column tiles start at multiples of 64 columns.  A box class and a box
     with holes that start and end mid-row, and the remaining interior, are each
     updated exactly once and only inside their class; the class reading its own
-    cells at time 0 runs compute-then-store."""
+    cells at time 0 runs compute-then-store.  Rows of 259 and 300 columns hold
+    two full 128-column tiles and a partial one: an odd row pitch reads the
+    staged columns one element at a time, an even one as 16-byte pairs.
+
+    The oracle runs sequentially here.  Class 4 reads u[0, 0, 1] of its own class
+    at time 0, and the reference's OpenMP nest (parallel for collapse(3), in
+    place) races where one thread's chunk of cells ends: that thread reads the
+    next cell after or before its owner rewrote it, depending on timing and the
+    thread count.  Sequential order reads the cell before it is rewritten, which
+    is what compute-then-store gives on the device."""
     @xgrid.kernel()
     def k(u: f3, v: f3) -> None:
         u[0, 0, 0] = v[0, 0, 1] + v[-1, 0, 0] * 0.5 + v[0, 1, 0] * 0.25
@@ -317,7 +329,7 @@ def test_3d_classes_at_any_column(double, shape):
     sub[...] = np.where(holes, 0, 4)                                   # mid-row, with holes
     du, hu = grid_pair(xgrid, shape, float, mask, rng.random(shape))
     dv, hv = grid_pair(xgrid, shape, float, None, rng.random(shape))
-    run_pair(k, (du, dv), (hu, hv), steps=3)
+    run_pair(k, (du, dv), (hu, hv), steps=3, parallel=False)
     assert du.class_info(2)["kind"] == "box" and du.class_info(2)["lo"][2] == 5
     assert du.class_info(4)["kind"] == "masked"
     assert_bitexact(du, hu)
@@ -464,13 +476,19 @@ def test_division_by_scalar_is_ieee_for_every_double(double):
     vals[: n // 2] = rng.random(n // 2) * 10.0 ** rng.integers(-30, 30, n // 2)
     vals[::3] = np.negative(vals[::3])             # a sign flip (NaNs included)
     vals[:8] = [0.0, -0.0, np.inf, -np.inf, np.nan, 5e-324, 1.7976931348623157e308, 2.0 ** -900]
+    # numerators near 2^+-500
+    vals[8:16] = [1.5 * 2.0 ** 500, 1.5 * 2.0 ** 501, 1.5 * 2.0 ** -500, 1.5 * 2.0 ** -501,
+                  -(2.0 ** 500), -(2.0 ** -500), np.nextafter(2.0 ** 501, 0), np.nextafter(2.0 ** -500, 0)]
     a = xgrid.Grid((n,), float)
     a.now[:] = vals
     u = xgrid.Grid((n,), float)
     # powers of two take the reciprocal form (the largest, 2^1023, has the subnormal
     # reciprocal 2^-1023); the subnormal 2^-1023 itself and the other divisors divide
+    # (all-ones and near-one significands, very large and very small ones among them)
     for c in (3.0, 0.1, -7.25, 1e-5, 6.02e23, 2.0 ** -120, 2.0 ** 150, -1e300, 0.0,
-              1.0, -0.5, 2.0 ** -10, 2.0 ** -18, 2.0 ** -1022, -(2.0 ** 1023), 2.0 ** 1000, 2.0 ** -1023, np.inf):
+              1.0, -0.5, 2.0 ** -10, 2.0 ** -18, 2.0 ** -1022, -(2.0 ** 1023), 2.0 ** 1000, 2.0 ** -1023, np.inf,
+              np.nextafter(2.0, 0), np.nextafter(1.0, 2), 1.5 * 2.0 ** 400, 1.5 * 2.0 ** 401,
+              1.5 * 2.0 ** -400, 1.5 * 2.0 ** -401, 3e-300, 7e250, 2.0 / 2048 ** 2 * 3.0):
         divk(u, a, c)
         with np.errstate(all="ignore"):
             want = vals / c
@@ -494,11 +512,16 @@ def test_division_by_scalar_is_ieee_for_every_float(single):
     vals[: n // 2] = (rng.random(n // 2) * 10.0 ** rng.integers(-20, 20, n // 2)).astype(np.float32)
     vals[::3] = np.negative(vals[::3])
     vals[:8] = np.array([0.0, -0.0, np.inf, -np.inf, np.nan, 1e-45, 3.4028235e38, 2.0 ** -100], np.float32)
+    # numerators near 2^+-60
+    vals[8:14] = np.array([1.5 * 2.0 ** 60, 1.5 * 2.0 ** 61, 1.5 * 2.0 ** -60, 1.5 * 2.0 ** -61,
+                           -(2.0 ** 60), -(2.0 ** -60)], np.float32)
     a = xgrid.Grid((n,), float)
     a.now[:] = vals
     u = xgrid.Grid((n,), float)
     assert a.now.dtype == np.float32
-    for c in (3.0, 0.1, -7.25, 2.0 ** -3, 2.0 ** 20, -(2.0 ** 127), 2.0 ** -126, 2.0 ** -127, 1.0, 0.0):
+    for c in (3.0, 0.1, -7.25, 2.0 ** -3, 2.0 ** 20, -(2.0 ** 127), 2.0 ** -126, 2.0 ** -127, 1.0, 0.0,
+              float(np.nextafter(np.float32(2), np.float32(0))), float(np.nextafter(np.float32(1), np.float32(2))),
+              1.5 * 2.0 ** 40, 1.5 * 2.0 ** 41, 1.5 * 2.0 ** -40, 1.5 * 2.0 ** -41, 1e-30, 7e30, 1.1):
         cf = np.float32(c)
         divf(u, a, float(cf))
         with np.errstate(all="ignore"):

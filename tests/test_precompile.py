@@ -68,3 +68,37 @@ def test_precompile_can_be_switched_off_and_requested(tmp_path):
     assert k._precompiled is None
     (h,) = xgrid.precompile(k, wait=True)
     assert h.done() and h.error is None and k._precompiled is h
+
+
+def test_concurrent_stub_compiles_of_one_source_all_load(tmp_path):
+    """Threads of one process compiling the same host stub at once (a background
+    precompile and a batch build, say) each get a complete library: every
+    compile writes files of its own and renames them into place."""
+    import ctypes
+    import threading
+    from xgrid_amd.compilers import compile_host
+    src = "".join(f"double xg_f{i}(double x) {{ return x * {i}.5 + {i}; }}\n" for i in range(400))
+    errors, paths = [], []
+    start = threading.Barrier(8)
+
+    def work():
+        try:
+            start.wait()
+            p = compile_host(src, str(tmp_path), ["gcc", "clang"])
+            ctypes.CDLL(p)
+            paths.append(p)
+        except Exception as e:         # collected and reported below
+            errors.append(repr(e))
+
+    ts = [threading.Thread(target=work) for _ in range(8)]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join()
+    assert not errors, errors
+    assert len(set(paths)) == 1
+    lib = ctypes.CDLL(paths[0])
+    lib.xg_f3.restype = ctypes.c_double
+    lib.xg_f3.argtypes = [ctypes.c_double]
+    assert lib.xg_f3(2.0) == 2.0 * 3.5 + 3
+    assert not [f for f in os.listdir(os.path.dirname(paths[0])) if ".tmp" in f or f.startswith("tmp")]

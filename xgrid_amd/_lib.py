@@ -33,7 +33,7 @@ class StmtDesc(Structure):
         ("ref_arg", c_int32 * XG_MAX_REFS), ("ref_level", c_int32 * XG_MAX_REFS),
         ("store_off", c_int32 * 3), ("nhazard", c_int32), ("hazard", (c_int32 * 3) * XG_MAX_HAZ),
         ("param_bytes", c_int32), ("overstep", c_int32), ("group", c_int32), ("group_pos", c_int32),
-        ("pair_fwd", c_int32), ("pair_pad", c_int32),
+        ("pair_fwd", c_int32), ("tile3_cols", c_int32),
         ("fn_main", c_void_p), ("fn_pingpong", c_void_p), ("fn_phase2", c_void_p),
         ("fn_main_b", c_void_p), ("fn_pingpong_b", c_void_p), ("fn_phase2_b", c_void_p),
         ("fn_pair", c_void_p),

@@ -139,6 +139,7 @@ class HipKernel:
             d.fn_phase2_b = fn(m.phase2 + "_b")
             d.fn_pair = fn(m.pair) if m.pair else None
             d.pair_fwd = int(m.pair_fwd)
+            d.tile3_cols = m.tile3_cols
         groups = (_lib.GroupDesc * max(1, len(low.groups)))()
         for i, g in enumerate(low.groups):
             groups[i].first = g.first

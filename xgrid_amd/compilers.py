@@ -15,6 +15,7 @@ import hashlib
 import os
 import shutil
 import subprocess
+import tempfile
 import threading
 from ctypes import c_char_p
 
@@ -95,12 +96,20 @@ def compile_host(src: str, cacheroot: str, cc_candidates, opt_level: int = 2) ->
     lib = base + ".stub.so"
     if os.path.exists(lib):
         return lib
-    c_path = base + ".stub.c"
-    with open(c_path, "w") as f:
+    # Every call writes files of its own and renames them into place: a background
+    # precompile and a build in another thread of the same process may compile the
+    # same stub at once, and a shared temporary name would let one thread load the
+    # library while the other's linker is still writing it.
+    fd, c_tmp = tempfile.mkstemp(suffix=".stub.c", dir=os.path.dirname(base))
+    with os.fdopen(fd, "w") as f:
         f.write(src)
-    tmp = lib + f".tmp{os.getpid()}"
+    c_path = base + ".stub.c"
+    os.replace(c_tmp, c_path)
+    fd, tmp = tempfile.mkstemp(suffix=".stub.so.tmp", dir=os.path.dirname(base))
+    os.close(fd)
     proc = subprocess.run([cc, *flags[:-1], c_path, "-o", tmp, "-lm"], capture_output=True, text=True)
     if proc.returncode != 0:
+        os.unlink(tmp)
         raise CompileError(f"failed to compile '{c_path}' due to:\n{proc.stderr}")
     os.replace(tmp, lib)
     return lib

@@ -245,7 +245,7 @@ int run_stmt(xg_batch* b, int32_t sid, const std::vector<int>& idx, const std::v
     H.dom.flags = uni;
     if (mode == XG_MODE_INPLACE) {
       fill_view(H.out, G, G->levels[0]);
-      add_entry(b, inplace, k, d, H, par[t], blocks_for_dom(d.dims, H.dom), &rc);
+      add_entry(b, inplace, k, d, H, par[t], blocks_for_dom(d.dims, H.dom, tile3_cols(d)), &rc);
     } else if (mode == XG_MODE_PINGPONG) {
       if (H.dom.kind == XG_DOM_LIST) H.dom.kind = XG_DOM_MASKED;
       H.dom.flags = XG_F_PINGPONG | uni;
@@ -259,7 +259,7 @@ int run_stmt(xg_batch* b, int32_t sid, const std::vector<int>& idx, const std::v
       H.dom.flags = XG_F_TO_TMP | uni;
       fill_view(H.out, G, G->levels[0]);
       fill_view(H.src, G, G->scratch);
-      add_entry(b, ph1, k, d, H, par[t], blocks_for_dom(d.dims, H.dom), &rc);
+      add_entry(b, ph1, k, d, H, par[t], blocks_for_dom(d.dims, H.dom, tile3_cols(d)), &rc);
       xg_launch_hdr H2 = H;
       H2.dom.flags = 0;
       for (int a = 0; a < 3; ++a) H2.dom.so[a] = d.store_off[a];

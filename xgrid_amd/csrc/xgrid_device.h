@@ -61,6 +61,10 @@ template <class T>
 __device__ __forceinline__ gptr<T> gp(const void* p) {
   return (gptr<T>)(p);
 }
+/* Two adjacent 8-byte elements read as one 16-byte access (staged 3-D loads;
+ * only at addresses that are multiples of 16). */
+template <class T>
+using vec2 = T __attribute__((ext_vector_type(2)));
 /* The views and parameter blocks a batch table points to are written by the
  * host before the launch and only read by kernels: reading them through the
  * constant address space lets uniform reads be scalar loads into SGPRs, as
@@ -196,10 +200,15 @@ __device__ __forceinline__ void unlin(const int32_t* shape, int64_t c, int& i0, 
 }
 
 /* This workgroup's tile of the box [lo, hi): the NC cells of this work-item,
- * whether each is inside the box, and the tile's extent clipped to the box. */
-template <int D>
+ * whether each is inside the box, and the tile's extent clipped to the box.
+ * CPL (3-D only): columns per work-item.  With CPL = 2 a tile is 128 columns
+ * wide and work-item (r, c) holds columns 2c and 2c + 1 of each of its planes:
+ * cell n is plane n / 2, column 2c + n % 2 (statements whose staged loads read
+ * two columns per lane). */
+template <int D, int CPL = 1>
 struct TileCells {
-  static constexpr int NC = Tile<D>::NC;
+  static constexpr int NC = Tile<D>::NC * (D == 3 ? CPL : 1);
+  static constexpr int E2 = Tile<D>::E2 * (D == 3 ? CPL : 1);
   int c[NC][3];
   bool in[NC];
   int32_t tlo[3], thi[3];
@@ -215,7 +224,7 @@ struct TileCells {
      * a line is fetched once instead of in parts by two tiles (512^3 FP32:
      * 312 against 352 us).  Columns of the first tile below lo[2] are not in
      * the tile. */
-    const int a2 = D > 2 ? (lo[2] & ~(TL::E2 - 1)) : 0;
+    const int a2 = D > 2 ? (lo[2] & ~(E2 - 1)) : 0;
     if (D == 2) {
       const int n1 = (hi[1] - lo[1] + TL::E1 - 1) / TL::E1;
       t0 = (int)tile / n1;
@@ -229,7 +238,7 @@ struct TileCells {
        * bands slowest, the reuse distance of a halo row was a whole band
        * (4 MB there): 282 against 312 us with aligned columns. */
       constexpr int64_t G = 16;
-      const int64_t n2 = (hi[2] - a2 + TL::E2 - 1) / TL::E2;
+      const int64_t n2 = (hi[2] - a2 + E2 - 1) / E2;
       const int64_t n1 = (hi[1] - lo[1] + TL::E1 - 1) / TL::E1;
       const int64_t n0 = (hi[0] - lo[0] + TL::E0 * TL::CH - 1) / (TL::E0 * TL::CH);
       const int64_t per = G * n0 * n2;
@@ -244,12 +253,12 @@ struct TileCells {
     }
     tlo[0] = lo[0] + t0 * (TL::E0 * TL::CH) + chunk * TL::E0;
     tlo[1] = D > 1 ? lo[1] + t1 * TL::E1 : 0;
-    const int c2 = a2 + t2 * TL::E2;  /* first column of the (aligned) 3-D tile */
+    const int c2 = a2 + t2 * E2;      /* first column of the (aligned) 3-D tile */
     col0 = c2;
     tlo[2] = D > 2 ? max(c2, lo[2]) : 0;
     thi[0] = min(tlo[0] + TL::E0, hi[0]);
     thi[1] = D > 1 ? min(tlo[1] + TL::E1, hi[1]) : 1;
-    thi[2] = D > 2 ? min(c2 + TL::E2, hi[2]) : 1;
+    thi[2] = D > 2 ? min(c2 + E2, hi[2]) : 1;
 #pragma unroll
     for (int n = 0; n < NC; ++n) {
       if (D == 1) {
@@ -261,9 +270,9 @@ struct TileCells {
         c[n][1] = tlo[1] + (tid & 63);
         c[n][2] = 0;
       } else {
-        c[n][0] = tlo[0] + n;
+        c[n][0] = tlo[0] + n / CPL;
         c[n][1] = tlo[1] + (tid >> 6);
-        c[n][2] = c2 + (tid & 63);
+        c[n][2] = c2 + (tid & 63) * CPL + n % CPL;
       }
       in[n] = c[n][0] < thi[0] && (D < 2 || c[n][1] < thi[1]) && (D < 3 || (c[n][2] >= tlo[2] && c[n][2] < thi[2]));
     }
@@ -298,10 +307,11 @@ __device__ __forceinline__ void box_cells(const xg_launch_hdr& H, const typename
                                           const typename S::Params& P, uint32_t tile, unsigned char* st3_lds) {
   constexpr int D = S::D;
   typedef typename S::T T;
+  typedef TileCells<D, S::CPL> TC;
   for (int ch = 0; ch < Tile<D>::CH; ++ch) {
-    TileCells<D> tc(H.dom.lo, H.dom.hi, tile, ch);
+    TC tc(H.dom.lo, H.dom.hi, tile, ch);
     if (tc.tlo[0] >= H.dom.hi[0]) break;
-    constexpr int NC = TileCells<D>::NC;
+    constexpr int NC = TC::NC;
     /* A work-item evaluates all its cells without branching, so their loads are
      * in flight together: a cell past the tile's edge is evaluated at the
      * nearest cell inside the tile (a valid address) and not stored. */
@@ -312,6 +322,7 @@ __device__ __forceinline__ void box_cells(const xg_launch_hdr& H, const typename
         tc.in[n] = tc.in[n] && gp<const int32_t>(H.dom.mask)[lin<D>(H.out.shape, tc.c[n][0], tc.c[n][1], tc.c[n][2])] == H.dom.klass;
     }
     T v[NC];
+    bool pairs = false;   /* store a lane's two columns with one 16-byte store */
     if (tile_safe<S>(H, tc.tlo, tc.thi)) {
       if constexpr (D == 3 && S::ST3) {
         /* Staged 3-D tile: each array's columns of the tile's planes are read
@@ -323,10 +334,16 @@ __device__ __forceinline__ void box_cells(const xg_launch_hdr& H, const typename
         S::st3_load(st, st3_lds, R, H.out.shape, tc.tlo[0], tc.tlo[1], tc.col0);
 #pragma unroll
         for (int n = 0; n < NC; ++n) {
-          S::st3_plane_n(st, n);
+          if (n % S::CPL == 0) S::st3_plane_n(st, n / S::CPL);   /* a plane's neighbours, once */
           v[n] = S::value_st3(st, R, P, H.out.shape, n, tc.c[n][0], tc.c[n][1], tc.c[n][2]);
         }
         if (Tile<D>::CH > 1) __syncthreads();  /* the next chunk rewrites the LDS copy */
+        /* In a safe tile every stored cell lies inside the grid, so with no store
+         * offset and no scratch destination the store is a plain one at the cell's
+         * index, and a lane's two adjacent columns are one 16-byte store where the
+         * base is 16-byte aligned and the row pitch even. */
+        if constexpr (S::CPL == 2 && sizeof(T) == 8 && S::SO0 == 0 && S::SO1 == 0 && S::SO2 == 0)
+          pairs = !(H.dom.flags & XG_F_TO_TMP) && ((uint64_t)H.out.ptr & 15) == 0 && (H.out.shape[2] & 1) == 0;
       } else {
 #pragma unroll
         for (int n = 0; n < NC; ++n) v[n] = S::template value<true>(R, P, H.out.shape, tc.c[n][0], tc.c[n][1], tc.c[n][2]);
@@ -334,6 +351,23 @@ __device__ __forceinline__ void box_cells(const xg_launch_hdr& H, const typename
     } else {
 #pragma unroll
       for (int n = 0; n < NC; ++n) v[n] = S::template value<false>(R, P, H.out.shape, tc.c[n][0], tc.c[n][1], tc.c[n][2]);
+    }
+    if constexpr (D == 3 && S::CPL == 2 && sizeof(T) == 8) {
+      if (pairs) {
+#pragma unroll
+        for (int n = 0; n < NC; n += 2) {
+          if (tc.in[n] && tc.in[n + 1]) {
+            vec2<T> w;
+            w.x = v[n];
+            w.y = v[n + 1];
+            *(gptr<vec2<T>>)(gp<T>(H.out.ptr) + lin<3>(H.out.shape, tc.c[n][0], tc.c[n][1], tc.c[n][2])) = w;
+          } else {
+            if (tc.in[n]) store<S, OV>(H, tc.c[n][0], tc.c[n][1], tc.c[n][2], v[n]);
+            if (tc.in[n + 1]) store<S, OV>(H, tc.c[n + 1][0], tc.c[n + 1][1], tc.c[n + 1][2], v[n + 1]);
+          }
+        }
+        continue;
+      }
     }
 #pragma unroll
     for (int n = 0; n < NC; ++n)

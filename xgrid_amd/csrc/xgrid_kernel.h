@@ -109,8 +109,11 @@ void fill_dom(xg_dom& dom, const ClassPlan& p, const xg_grid* G, int32_t klass);
 void fill_view(xg_gview& v, const xg_grid* G, void* ptr);
 bool uniform_refs(xg_kernel* k, const xg_stmt_desc& d);
 int fill_refs(xg_kernel* k, const xg_stmt_desc& d, std::vector<xg_gview>& refs);
-int64_t tiles_for(int D, const int32_t* lo, const int32_t* hi);
-int64_t blocks_for_dom(int D, const xg_dom& dom);
+// Workgroups over the box [lo, hi) for a statement's tile; cols3 = columns of a 3-D tile.
+int64_t tiles_for(int D, const int32_t* lo, const int32_t* hi, int cols3 = 64);
+int64_t blocks_for_dom(int D, const xg_dom& dom, int cols3 = 64);
+// The 3-D tile width of a statement's run_main kernels (fn_main, fn_main_b).
+int tile3_cols(const xg_stmt_desc& d);
 int ensure_scratch(xg_kernel* k);
 // Can loop `lid` run blocked on the bound grids (aliasing), and is its plan for
 // the grid's current mask known?  Resets the plan when the grid or mask changed.

@@ -195,17 +195,20 @@ int fill_refs(xg_kernel* k, const xg_stmt_desc& d, std::vector<xg_gview>& refs) 
 }
 
 // workgroups covering the box [lo, hi) with the tile of a D-dimensional statement
-int64_t tiles_for(int D, const int32_t* lo, const int32_t* hi) {
+int64_t tiles_for(int D, const int32_t* lo, const int32_t* hi, int cols3) {
   if (D == 1) return ceil_div(hi[0] - lo[0], 1024);
   if (D == 2) return ceil_div(hi[0] - lo[0], 4) * ceil_div(hi[1] - lo[1], 64);
-  // column tiles start at multiples of 64 columns (TileCells in xgrid_device.h)
-  return ceil_div(hi[0] - lo[0], 4 * XG_TILE3_CHUNKS) * ceil_div(hi[1] - lo[1], 4) * ceil_div(hi[2] - (lo[2] & ~63), 64);
+  // column tiles of cols3 (64 or 128) columns start at multiples of cols3 (TileCells in xgrid_device.h)
+  return ceil_div(hi[0] - lo[0], 4 * XG_TILE3_CHUNKS) * ceil_div(hi[1] - lo[1], 4) *
+         ceil_div(hi[2] - (lo[2] & ~(cols3 - 1)), cols3);
 }
 
-int64_t blocks_for_dom(int D, const xg_dom& dom) {
+int64_t blocks_for_dom(int D, const xg_dom& dom, int cols3) {
   if (dom.kind == XG_DOM_LIST) return std::min<int64_t>(ceil_div(dom.count, kThreads), 16384);
-  return tiles_for(D, dom.lo, dom.hi);
+  return tiles_for(D, dom.lo, dom.hi, cols3);
 }
+
+int tile3_cols(const xg_stmt_desc& d) { return d.tile3_cols == 128 ? 128 : 64; }
 
 int do_launch(xg_kernel* k, int32_t sid, void* fn, int64_t blocks, int threads, void** args) {
   if (blocks <= 0) return XG_OK;
@@ -245,7 +248,7 @@ int exec_stmt(xg_kernel* k, int32_t sid, const void* params) {
   H.dom.flags = uni;
   if (mode == XG_MODE_INPLACE) {
     fill_view(H.out, G, G->levels[0]);
-    return do_launch(k, sid, d.fn_main, blocks_for_dom(d.dims, H.dom), kThreads, args);
+    return do_launch(k, sid, d.fn_main, blocks_for_dom(d.dims, H.dom, tile3_cols(d)), kThreads, args);
   }
   if (!G->scratch) return fail(XG_ERR_STATE, "scratch buffer missing");
   if (mode == XG_MODE_PINGPONG) {
@@ -266,7 +269,7 @@ int exec_stmt(xg_kernel* k, int32_t sid, const void* params) {
   H.dom.flags = XG_F_TO_TMP | uni;
   fill_view(H.out, G, G->levels[0]);
   fill_view(H.src, G, G->scratch);
-  rc = do_launch(k, sid, d.fn_main, blocks_for_dom(d.dims, H.dom), kThreads, args);
+  rc = do_launch(k, sid, d.fn_main, blocks_for_dom(d.dims, H.dom, tile3_cols(d)), kThreads, args);
   if (rc) return rc;
   xg_launch_hdr H2 = H;
   H2.dom.flags = 0;

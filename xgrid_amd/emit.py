@@ -86,6 +86,7 @@ class StmtMeta:
     pair_fwd: bool = False        # ... with the next one's loads of this store (own cell, time 0) forwarded
     fwd_from: str | None = None   # (second of a forwarding pair) the grid whose time-0 loads are forwarded
     hoisted: list = field(default_factory=list)   # [Hoist]: launch-uniform subexpressions computed by the host
+    tile3_cols: int = 64          # columns of a 3-D tile of this statement's run_main kernels
 
 
 @dataclass
@@ -366,22 +367,29 @@ def function_symbol(op) -> str:
 
 
 class _St3Plan:
-    """Staged loads of a 3-D statement over one tile (4 planes x 4 rows x 64
-    columns, work-item = (row wave, column lane), 4 planes per work-item), for
-    arrays of 8-byte elements.
+    """Staged loads of a 3-D statement over one tile (4 planes x 4 rows x 64 * CPL
+    columns, work-item = (row wave, column lane), CPL columns of 4 planes per
+    work-item), for arrays of 8-byte elements.
 
     Per array j (a distinct (grid, level) of the statement) the columns of the
-    planes the statement reads are held in registers `q{j}[u]`, plane
-    p0 + pmin + u.  A load at offset (o0, 0, 0) reads q{j}[n + o0 - pmin]; at
-    (o0, 0, +-1) the neighbouring lane's value, moved with a DPP lane shift
-    (lanes 0 and 63 read their outer neighbour from memory); at (o0, +-1, 0)
-    a row of an LDS copy of the tile's 4 rows plus one halo row on each side.
-    Other loads (diagonals, larger offsets, lower-dimensional grids) stay
-    global loads.  Only used in tiles whose cells and reach lie inside the grid,
-    so every value a stored cell uses is exact; padding lanes and rows read
-    clamped addresses and are not stored."""
+    planes the statement reads are held in registers `q{j}[u][k]`, plane
+    p0 + pmin + u, the lane's k-th column.  A load at offset (o0, 0, 0) reads
+    q{j}[n + o0 - pmin][k]; at (o0, 0, +-1) the lane's other column or the
+    neighbouring lane's, moved with a DPP lane shift (lanes 0 and 63 read their
+    outer neighbour from memory); at (o0, +-1, 0) a row of an LDS copy of the
+    tile's 4 rows plus one halo row on each side.  Other loads (diagonals,
+    larger offsets, lower-dimensional grids) stay global loads.  Only used in
+    tiles whose cells and reach lie inside the grid, so every value a stored
+    cell uses is exact; padding lanes and rows read clamped addresses and are
+    not stored.
 
-    NP, ROWS, COLS = 4, 4, 64
+    Two columns per lane (the default) make a wave read 1 KB of a row at a
+    time instead of 512 B: the 512^3 FP64 7-point update measured 376 us
+    against 534 us with one column in the same probe.  Each lane reads its
+    two columns with two 8-byte loads, so no alignment beyond the element's
+    is assumed.  XGRID_STAGE3_COLS=1 selects one column per lane."""
+
+    NP, ROWS = 4, 4
     MAX_WINDOW = 8          # most planes held in registers per array
     MAX_LDS_GROUPS = 3      # most (array, plane offset) pairs staged in LDS
 
@@ -392,6 +400,8 @@ class _St3Plan:
         self.xg = []                # [(j, o0)] with a column-neighbour load
         self.xside = {}             # (j, o0) -> set of +-1
         self.yg = []                # [(j, o0)] with a row-neighbour load
+        self.cpl = 1 if os.environ.get("XGRID_STAGE3_COLS", "2") == "1" else 2
+        self.COLS = 64 * self.cpl   # columns of a tile
 
     def add(self, j, off) -> bool:
         o0, o1, o2 = off
@@ -429,15 +439,25 @@ class _St3Plan:
         return max(self.planes[j]) - self.pmin(j) + self.NP
 
     def access(self, j, off):
+        """The staged value of cell n's load at `off` (n = plane * CPL + column)."""
         if j not in self.planes:
             return None
         o0, o1, o2 = off
+        pl = "n" if self.cpl == 1 else f"(n / {self.cpl})"
+        k = "0" if self.cpl == 1 else f"(n % {self.cpl})"
+        q = f"st.q{j}[{pl} + {o0 - self.pmin(j)}]"
         if o1 == 0 and o2 == 0:
-            return f"st.q{j}[n + {o0 - self.pmin(j)}]"
+            return f"{q}[{k}]"
         if o1 == 0 and abs(o2) == 1 and (j, o0) in self.xg:
-            return f"st.x{'p' if o2 > 0 else 'n'}{self.xg.index((j, o0))}"
+            g = self.xg.index((j, o0))
+            if self.cpl == 1:
+                return f"st.x{'p' if o2 > 0 else 'n'}{g}"
+            # the lane's other column, or the neighbouring lane's
+            if o2 > 0:
+                return f"({k} == {self.cpl - 1} ? st.xp{g} : {q}[{k} + 1])"
+            return f"({k} == 0 ? st.xn{g} : {q}[{k} - 1])"
         if abs(o1) == 1 and o2 == 0 and (j, o0) in self.yg:
-            return f"st.y{'p' if o1 > 0 else 'n'}{self.yg.index((j, o0))}"
+            return f"st.y{'p' if o1 > 0 else 'n'}{self.yg.index((j, o0))}[{k}]"
         return None
 
     def lds_bytes(self) -> int:
@@ -450,37 +470,61 @@ class _St3Plan:
     def members(self, expr: str) -> list:
         """st3_load: the plane registers, the edge lanes' outer columns and the
         LDS copy (ends with a barrier); st3_plane_n: the neighbours of plane
-        n, formed just before its value so that only one plane's are live;
+        n, formed just before its cells so that only one plane's are live;
         value_st3: the statement's expression over them."""
-        NP, R2, C = self.NP, self.ROWS + 2, self.COLS
-        out = ["  static constexpr bool ST3 = true;", f"  static constexpr int ST3_LDS = {max(16, self.lds_bytes())};"]
+        NP, R2, C, K = self.NP, self.ROWS + 2, self.COLS, self.cpl
+        out = ["  static constexpr bool ST3 = true;", f"  static constexpr int ST3_LDS = {max(16, self.lds_bytes())};",
+               f"  static constexpr int CPL = {K};"]
         fields = []
         for j in sorted(self.planes):
-            fields.append(f"{self.types[j]} q{j}[{self.window(j)}];")
+            fields.append(f"{self.types[j]} q{j}[{self.window(j)}][{K}];")
         for g, (j, o0) in enumerate(self.xg):
             fields.append(f"{self.types[j]} e{g}[{NP}];")
             for sd in sorted(self.xside[(j, o0)]):
                 fields.append(f"{self.types[j]} x{'p' if sd > 0 else 'n'}{g};")
         for g, (j, o0) in enumerate(self.yg):
-            fields.append(f"{self.types[j]} yp{g}; {self.types[j]} yn{g};")
+            fields.append(f"{self.types[j]} yp{g}[{K}]; {self.types[j]} yn{g}[{K}];")
         fields.append("const unsigned char* lds; int lane, wr;")
         out.append(f"  struct St3 {{ {' '.join(fields)} }};")
         out.append("  __device__ static __forceinline__ void st3_load(St3& st, unsigned char* lds, const Refs& R, "
                    "const int32_t* sh, int p0, int r0, int c0) {")
+        # In a tile whose columns all lie inside the row (`full`: every tile but a
+        # partial last one) a lane's columns need no clamp; where their address is a
+        # multiple of 16 bytes they are read as one 16-byte load.  (Marking the wave's
+        # row uniform with readfirstlane made the FP64 heat kernel spill.)
         out.append("    const int lane = threadIdx.x & 63, wr = threadIdx.x >> 6;")
         out.append("    st.lds = lds; st.lane = lane; st.wr = wr;")
         out.append("    const int64_t P1 = sh[2], P0 = (int64_t)sh[1] * sh[2];")
-        out.append("    const int64_t row = min(r0 + wr, sh[1] - 1), col = min(c0 + lane, sh[2] - 1);")
-        out.append("    (void)lds; (void)P1; (void)P0;")
+        out.append("    const int64_t row = min(r0 + wr, sh[1] - 1);")
+        out.append(f"    const int cl = c0 + lane * {K};")
+        out.append(f"    const bool full = c0 + {C} <= sh[2];")
+        out.append("    (void)lds; (void)P1; (void)P0; (void)cl; (void)full;")
         for j in sorted(self.planes):
             t = self.types[j]
             out.append(f"    const xg::gptr<const {t}> g{j} = xg::gp<const {t}>(R.v[{j}].ptr);")
+            if K == 2:
+                # pairs at 16-byte aligned addresses: a 16-byte aligned base, an even row
+                # pitch (then every plane pitch is even) and even columns
+                out.append(f"    if (full && ((uint64_t)R.v[{j}].ptr & 15) == 0 && (P1 & 1) == 0) {{")
+                out.append("#pragma unroll")
+                out.append(f"      for (int u = 0; u < {self.window(j)}; ++u) {{")
+                out.append(f"        const xg::gptr<const {t}> rp = g{j} + (xg::st3_plane(p0 + u + {self.pmin(j)}, sh) * P0 + row * P1);")
+                out.append(f"        const xg::vec2<{t}> v = *(xg::gptr<const xg::vec2<{t}>>)(rp + cl);")
+                out.append(f"        st.q{j}[u][0] = v.x; st.q{j}[u][1] = v.y;")
+                out.append("      }")
+                out.append("    } else {")
+            else:
+                out.append("    {")
             out.append("#pragma unroll")
-            out.append(f"    for (int u = 0; u < {self.window(j)}; ++u) "
-                       f"st.q{j}[u] = g{j}[xg::st3_plane(p0 + u + {self.pmin(j)}, sh) * P0 + row * P1 + col];")
+            out.append(f"      for (int u = 0; u < {self.window(j)}; ++u) {{")
+            out.append(f"        const xg::gptr<const {t}> rp = g{j} + (xg::st3_plane(p0 + u + {self.pmin(j)}, sh) * P0 + row * P1);")
+            out.append("#pragma unroll")
+            out.append(f"        for (int k = 0; k < {K}; ++k) st.q{j}[u][k] = rp[min(cl + k, sh[2] - 1)];")
+            out.append("      }")
+            out.append("    }")
         if self.xg:
             out.append("    const bool edge = lane == 0 || lane == 63;")
-            out.append("    const int64_t ecol = lane == 0 ? max(c0 - 1, 0) : min(c0 + 64, sh[2] - 1);")
+            out.append(f"    const int64_t ecol = lane == 0 ? max(c0 - 1, 0) : min(c0 + {C}, sh[2] - 1);")
         for g, (j, o0) in enumerate(self.xg):
             t = self.types[j]
             out.append("#pragma unroll")
@@ -493,14 +537,33 @@ class _St3Plan:
             self._offs.append(off)
             out.append(f"    {{ {t}* L = ({t}*)(lds + {off});")
             out.append("#pragma unroll")
-            out.append(f"      for (int n = 0; n < {NP}; ++n) L[(n * {R2} + wr + 1) * {C} + lane] = "
-                       f"st.q{j}[n + {o0 - self.pmin(j)}];")
+            out.append(f"      for (int n = 0; n < {NP}; ++n)")
             out.append("#pragma unroll")
-            out.append("      for (int h = 0; h < 2; ++h) {")
-            out.append(f"        const int e = threadIdx.x + h * 256, hr = e >> 8, pl = (e >> 6) & {NP - 1};")
+            out.append(f"        for (int k = 0; k < {K}; ++k) L[(n * {R2} + wr + 1) * {C} + lane * {K} + k] = "
+                       f"st.q{j}[n + {o0 - self.pmin(j)}][k];")
+            # the halo rows above and below the tile: 2 x NP rows of C columns over 256 work-items
+            if K == 2:
+                # two columns per work-item, one 16-byte load, where aligned as above
+                out.append(f"      if (full && ((uint64_t)R.v[{j}].ptr & 15) == 0 && (P1 & 1) == 0) {{")
+                out.append("#pragma unroll")
+                out.append(f"        for (int h = 0; h < {2 * NP * C // 512}; ++h) {{")
+                out.append(f"          const int e = threadIdx.x + h * 256, cc = (e % {C // 2}) * 2, "
+                           f"pl = (e / {C // 2}) % {NP}, hr = e / {C // 2 * NP};")
+                out.append(f"          const int64_t gr = min(max(hr ? r0 + {self.ROWS} : r0 - 1, 0), sh[1] - 1);")
+                out.append(f"          *(xg::vec2<{t}>*)(L + (pl * {R2} + (hr ? {R2 - 1} : 0)) * {C} + cc) = "
+                           f"*(xg::gptr<const xg::vec2<{t}>>)(g{j} + (xg::st3_plane(p0 + pl + {o0}, sh) * P0 + gr * P1 + c0 + cc));")
+                out.append("        }")
+                out.append("      } else {")
+            else:
+                out.append("      {")
+            out.append("#pragma unroll")
+            out.append(f"      for (int h = 0; h < {2 * NP * C // 256}; ++h) {{")
+            out.append(f"        const int e = threadIdx.x + h * 256, cc = e % {C}, pl = (e / {C}) % {NP}, "
+                       f"hr = e / {C * NP};")
             out.append(f"        const int64_t gr = min(max(hr ? r0 + {self.ROWS} : r0 - 1, 0), sh[1] - 1);")
-            out.append(f"        L[(pl * {R2} + (hr ? {R2 - 1} : 0)) * {C} + lane] = "
-                       f"g{j}[xg::st3_plane(p0 + pl + {o0}, sh) * P0 + gr * P1 + col];")
+            out.append(f"        L[(pl * {R2} + (hr ? {R2 - 1} : 0)) * {C} + cc] = "
+                       f"g{j}[xg::st3_plane(p0 + pl + {o0}, sh) * P0 + gr * P1 + min(c0 + cc, sh[2] - 1)];")
+            out.append("      }")
             out.append("      }")
             out.append("    }")
             off += NP * R2 * C * self._size(j)
@@ -514,19 +577,26 @@ class _St3Plan:
         out.append("    (void)st; (void)n;")
         for g, (j, o0) in enumerate(self.xg):
             t = self.types[j]
-            out.append(f"    {{ const {t} c = st.q{j}[n + {o0 - self.pmin(j)}];")
+            q = f"st.q{j}[n + {o0 - self.pmin(j)}]"
             if 1 in self.xside[(j, o0)]:
-                out.append(f"      const {t} a = xg::lane_shift<1>(c);")
-                out.append(f"      st.xp{g} = st.lane == 63 ? st.e{g}[n] : a;")
+                # right of the lane's last column: the next lane's first
+                out.append(f"    {{ const {t} cf = {q}[0];")
+                out.append(f"      const {t} a = xg::lane_shift<1>(cf);")
+                out.append(f"      st.xp{g} = st.lane == 63 ? st.e{g}[n] : a; }}")
             if -1 in self.xside[(j, o0)]:
-                out.append(f"      const {t} b = xg::lane_shift<-1>(c);")
-                out.append(f"      st.xn{g} = st.lane == 0 ? st.e{g}[n] : b;")
-            out.append("    }")
+                # left of the lane's first column: the previous lane's last
+                out.append(f"    {{ const {t} cl = {q}[{K - 1}];")
+                out.append(f"      const {t} b = xg::lane_shift<-1>(cl);")
+                out.append(f"      st.xn{g} = st.lane == 0 ? st.e{g}[n] : b; }}")
         for g, (j, o0) in enumerate(self.yg):
             t = self.types[j]
             out.append(f"    {{ const {t}* L = (const {t}*)(st.lds + {self._offs[g]});")
-            out.append(f"      st.yn{g} = L[(n * {R2} + st.wr) * {C} + st.lane]; "
-                       f"st.yp{g} = L[(n * {R2} + st.wr + 2) * {C} + st.lane]; }}")
+            out.append("#pragma unroll")
+            out.append(f"      for (int k = 0; k < {K}; ++k) {{")
+            out.append(f"        st.yn{g}[k] = L[(n * {R2} + st.wr) * {C} + st.lane * {K} + k];")
+            out.append(f"        st.yp{g}[k] = L[(n * {R2} + st.wr + 2) * {C} + st.lane * {K} + k];")
+            out.append("      }")
+            out.append("    }")
         out.append("  }")
         out.append("  __device__ static __forceinline__ T value_st3(const St3& st, const Refs& R, const Params& P, "
                    "const int32_t* sh, int n, int i0, int i1, int i2) {")
@@ -739,6 +809,10 @@ class Emitter:
         else:
             out.append("  static constexpr bool ST3 = false;")
             out.append("  static constexpr int ST3_LDS = 0;")
+            out.append("  static constexpr int CPL = 1;")
+        if meta is not None:
+            # the host sizes this statement's 3-D launches by its tile width
+            meta.tile3_cols = plan3.COLS if expr_st3 is not None else 64
         if fwd_from is not None:
             # the same expression with its time-0 loads of `fwd_from` at the cell taken from the
             # value the paired statement just computed for that cell
