@@ -55,8 +55,8 @@ typedef struct xg_stmt_desc {
   int32_t group;                             /* fusion group (-1: none) */
   int32_t group_pos;
   int32_t pair_fwd;                          /* fn_pair forwards this store to the next statement */
-  int32_t tile3_cols;                        /* 3-D fn_main: columns per tile (64, or 128 for staged loads
-                                                of two columns per lane); 0 means 64 */
+  int32_t tile3_cols;                        /* 3-D fn_main: columns per tile (64, or 128 / 256 for staged
+                                                loads of 16 bytes per lane); 0 means 64 */
   void* fn_main;                             /* hipFunction_t of the class kernel */
   void* fn_pingpong;                         /* hipFunction_t of the Jacobi sweep (PINGPONG) */
   void* fn_phase2;                           /* hipFunction_t storing a temporary (two-phase) */

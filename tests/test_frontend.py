@@ -199,10 +199,11 @@ def _stmt_struct(src: str, sid: int) -> str:
 
 
 def test_staged_3d_plan(double, monkeypatch):
-    """3-D statements whose loads of 8-byte arrays include column or row
-    neighbours are staged: plane registers, DPP column shifts, an LDS copy of
-    the tile's rows (only for row neighbours); diagonal loads stay global
-    loads; 2-D statements, 4-byte arrays and XGRID_STAGE3=0 keep plain loads."""
+    """3-D statements whose loads include column or row neighbours are staged:
+    plane registers, DPP column shifts, an LDS copy of the tile's rows (only for
+    row neighbours), 16 bytes of a row per lane (2 columns of FP64, 4 of FP32);
+    diagonal loads stay global loads; 2-D statements, XGRID_STAGE3=0 and, for
+    4-byte arrays, XGRID_STAGE3_F32=0 keep plain loads."""
     from xgrid_amd import workloads as W
     from xgrid_amd.backend import lower_kernel
     f3 = xgrid.grid[float, 3]
@@ -235,8 +236,14 @@ def test_staged_3d_plan(double, monkeypatch):
     monkeypatch.setenv("XGRID_STAGE3", "0")
     assert "ST3 = true" not in lower_kernel(W.make_heat_3d())[0].device_src
     monkeypatch.delenv("XGRID_STAGE3")
-    xgrid.init(precision="float")                                     # 4-byte arrays: plain loads
+    xgrid.init(precision="float")                                     # 4-byte arrays: 4 columns per lane
+    low = lower_kernel(W.make_heat_3d())[0]
+    s0 = _stmt_struct(low.device_src, 0)
+    assert "CPL = 4" in s0 and "ST3_LDS = 24576" in s0 and low.stmts[0].tile3_cols == 256
+    assert "xg::vecn<float, 4>" in s0
+    monkeypatch.setenv("XGRID_STAGE3_F32", "0")
     assert "ST3 = true" not in lower_kernel(W.make_heat_3d())[0].device_src
+    monkeypatch.delenv("XGRID_STAGE3_F32")
 
 
 def _dpp_in_masked_regions(asm: str) -> dict:

@@ -297,6 +297,19 @@ def test_list_and_masked_classes(double):
 
 @pytest.mark.parametrize("shape", [(9, 11, 150), (6, 7, 64), (5, 40, 200), (7, 9, 259), (6, 10, 300)])
 def test_3d_classes_at_any_column(double, shape):
+    _classes_at_any_column(shape)
+
+
+@pytest.mark.parametrize("shape", [(5, 9, 150), (6, 10, 600), (7, 9, 518)])
+def test_3d_classes_at_any_column_fp32(single, shape):
+    """The same classes over FP32 arrays, staged four columns per lane on
+    256-column tiles: rows of 600 columns (a multiple of 4: 16-byte loads and
+    stores) and 518 (one element at a time), each with two full tiles and a
+    partial one."""
+    _classes_at_any_column(shape)
+
+
+def _classes_at_any_column(shape):
     """3-D column tiles start at multiples of 64 columns.  A box class and a box
     with holes that start and end mid-row, and the remaining interior, are each
     updated exactly once and only inside their class; the class reading its own

@@ -61,10 +61,10 @@ template <class T>
 __device__ __forceinline__ gptr<T> gp(const void* p) {
   return (gptr<T>)(p);
 }
-/* Two adjacent 8-byte elements read as one 16-byte access (staged 3-D loads;
- * only at addresses that are multiples of 16). */
-template <class T>
-using vec2 = T __attribute__((ext_vector_type(2)));
+/* K adjacent elements read or written as one access (staged 3-D tiles: K
+ * elements of 16 bytes together, only at addresses that are multiples of 16). */
+template <class T, int K>
+using vecn = T __attribute__((ext_vector_type(K)));
 /* The views and parameter blocks a batch table points to are written by the
  * host before the launch and only read by kernels: reading them through the
  * constant address space lets uniform reads be scalar loads into SGPRs, as
@@ -322,7 +322,7 @@ __device__ __forceinline__ void box_cells(const xg_launch_hdr& H, const typename
         tc.in[n] = tc.in[n] && gp<const int32_t>(H.dom.mask)[lin<D>(H.out.shape, tc.c[n][0], tc.c[n][1], tc.c[n][2])] == H.dom.klass;
     }
     T v[NC];
-    bool pairs = false;   /* store a lane's two columns with one 16-byte store */
+    bool pairs = false;   /* store a lane's CPL columns with one 16-byte store */
     if (tile_safe<S>(H, tc.tlo, tc.thi)) {
       if constexpr (D == 3 && S::ST3) {
         /* Staged 3-D tile: each array's columns of the tile's planes are read
@@ -340,10 +340,10 @@ __device__ __forceinline__ void box_cells(const xg_launch_hdr& H, const typename
         if (Tile<D>::CH > 1) __syncthreads();  /* the next chunk rewrites the LDS copy */
         /* In a safe tile every stored cell lies inside the grid, so with no store
          * offset and no scratch destination the store is a plain one at the cell's
-         * index, and a lane's two adjacent columns are one 16-byte store where the
-         * base is 16-byte aligned and the row pitch even. */
-        if constexpr (S::CPL == 2 && sizeof(T) == 8 && S::SO0 == 0 && S::SO1 == 0 && S::SO2 == 0)
-          pairs = !(H.dom.flags & XG_F_TO_TMP) && ((uint64_t)H.out.ptr & 15) == 0 && (H.out.shape[2] & 1) == 0;
+         * index, and a lane's CPL adjacent columns (16 bytes) are one store where
+         * the base is 16-byte aligned and the row pitch a multiple of CPL. */
+        if constexpr (S::CPL * sizeof(T) == 16 && S::SO0 == 0 && S::SO1 == 0 && S::SO2 == 0)
+          pairs = !(H.dom.flags & XG_F_TO_TMP) && ((uint64_t)H.out.ptr & 15) == 0 && (H.out.shape[2] % S::CPL) == 0;
       } else {
 #pragma unroll
         for (int n = 0; n < NC; ++n) v[n] = S::template value<true>(R, P, H.out.shape, tc.c[n][0], tc.c[n][1], tc.c[n][2]);
@@ -352,18 +352,23 @@ __device__ __forceinline__ void box_cells(const xg_launch_hdr& H, const typename
 #pragma unroll
       for (int n = 0; n < NC; ++n) v[n] = S::template value<false>(R, P, H.out.shape, tc.c[n][0], tc.c[n][1], tc.c[n][2]);
     }
-    if constexpr (D == 3 && S::CPL == 2 && sizeof(T) == 8) {
+    if constexpr (D == 3 && S::CPL * sizeof(T) == 16) {
       if (pairs) {
+        constexpr int K = S::CPL;
 #pragma unroll
-        for (int n = 0; n < NC; n += 2) {
-          if (tc.in[n] && tc.in[n + 1]) {
-            vec2<T> w;
-            w.x = v[n];
-            w.y = v[n + 1];
-            *(gptr<vec2<T>>)(gp<T>(H.out.ptr) + lin<3>(H.out.shape, tc.c[n][0], tc.c[n][1], tc.c[n][2])) = w;
+        for (int n = 0; n < NC; n += K) {
+          bool all = true;
+#pragma unroll
+          for (int k = 0; k < K; ++k) all = all && tc.in[n + k];
+          if (all) {
+            vecn<T, K> w;
+#pragma unroll
+            for (int k = 0; k < K; ++k) w[k] = v[n + k];
+            *(gptr<vecn<T, K>>)(gp<T>(H.out.ptr) + lin<3>(H.out.shape, tc.c[n][0], tc.c[n][1], tc.c[n][2])) = w;
           } else {
-            if (tc.in[n]) store<S, OV>(H, tc.c[n][0], tc.c[n][1], tc.c[n][2], v[n]);
-            if (tc.in[n + 1]) store<S, OV>(H, tc.c[n + 1][0], tc.c[n + 1][1], tc.c[n + 1][2], v[n + 1]);
+#pragma unroll
+            for (int k = 0; k < K; ++k)
+              if (tc.in[n + k]) store<S, OV>(H, tc.c[n + k][0], tc.c[n + k][1], tc.c[n + k][2], v[n + k]);
           }
         }
         continue;

@@ -198,7 +198,7 @@ int fill_refs(xg_kernel* k, const xg_stmt_desc& d, std::vector<xg_gview>& refs) 
 int64_t tiles_for(int D, const int32_t* lo, const int32_t* hi, int cols3) {
   if (D == 1) return ceil_div(hi[0] - lo[0], 1024);
   if (D == 2) return ceil_div(hi[0] - lo[0], 4) * ceil_div(hi[1] - lo[1], 64);
-  // column tiles of cols3 (64 or 128) columns start at multiples of cols3 (TileCells in xgrid_device.h)
+  // column tiles of cols3 (64, 128 or 256) columns start at multiples of cols3 (TileCells in xgrid_device.h)
   return ceil_div(hi[0] - lo[0], 4 * XG_TILE3_CHUNKS) * ceil_div(hi[1] - lo[1], 4) *
          ceil_div(hi[2] - (lo[2] & ~(cols3 - 1)), cols3);
 }
@@ -208,7 +208,8 @@ int64_t blocks_for_dom(int D, const xg_dom& dom, int cols3) {
   return tiles_for(D, dom.lo, dom.hi, cols3);
 }
 
-int tile3_cols(const xg_stmt_desc& d) { return d.tile3_cols == 128 ? 128 : 64; }
+// 64 columns per lane-row of a 3-D tile times the columns each lane holds (1, 2 or 4)
+int tile3_cols(const xg_stmt_desc& d) { return d.tile3_cols == 128 || d.tile3_cols == 256 ? d.tile3_cols : 64; }
 
 int do_launch(xg_kernel* k, int32_t sid, void* fn, int64_t blocks, int threads, void** args) {
   if (blocks <= 0) return XG_OK;

@@ -26,7 +26,8 @@ from dataclasses import dataclass, field
 
 from . import ir
 from .analysis import KernelInfo, StencilStmt, captured_value_type, structures_in
-from .types import (Boolean, Floating, GridType, Ignore, Integer, Pointer, Structure, Value, Void)
+from .types import (Boolean, Floating, GridType, Ignore, Integer, Pointer, Structure, Value, Void,
+                    element_nbytes)
 
 
 class EmitError(RuntimeError):
@@ -369,46 +370,51 @@ def function_symbol(op) -> str:
 class _St3Plan:
     """Staged loads of a 3-D statement over one tile (4 planes x 4 rows x 64 * CPL
     columns, work-item = (row wave, column lane), CPL columns of 4 planes per
-    work-item), for arrays of 8-byte elements.
+    work-item), for the arrays whose elements have the statement's size (4 or
+    8 bytes).
 
     Per array j (a distinct (grid, level) of the statement) the columns of the
     planes the statement reads are held in registers `q{j}[u][k]`, plane
     p0 + pmin + u, the lane's k-th column.  A load at offset (o0, 0, 0) reads
-    q{j}[n + o0 - pmin][k]; at (o0, 0, +-1) the lane's other column or the
-    neighbouring lane's, moved with a DPP lane shift (lanes 0 and 63 read their
-    outer neighbour from memory); at (o0, +-1, 0) a row of an LDS copy of the
-    tile's 4 rows plus one halo row on each side.  Other loads (diagonals,
-    larger offsets, lower-dimensional grids) stay global loads.  Only used in
-    tiles whose cells and reach lie inside the grid, so every value a stored
-    cell uses is exact; padding lanes and rows read clamped addresses and are
-    not stored.
+    q{j}[n + o0 - pmin][k]; at (o0, 0, +-1) the lane's next or previous column,
+    or the neighbouring lane's, moved with a DPP lane shift (lanes 0 and 63 read
+    their outer neighbour from memory); at (o0, +-1, 0) a row of an LDS copy of
+    the tile's 4 rows plus one halo row on each side.  Other loads (diagonals,
+    larger offsets, lower-dimensional grids, arrays of another element size)
+    stay global loads.  Only used in tiles whose cells and reach lie inside the
+    grid, so every value a stored cell uses is exact; padding lanes and rows
+    read clamped addresses and are not stored.
 
-    Two columns per lane (the default) make a wave read 1 KB of a row at a
-    time instead of 512 B: the 512^3 FP64 7-point update measured 376 us
-    against 534 us with one column in the same probe.  Each lane reads its
-    two columns with two 8-byte loads, so no alignment beyond the element's
-    is assumed.  XGRID_STAGE3_COLS=1 selects one column per lane."""
+    A lane holds 16 bytes of each row (CPL = 2 columns of 8-byte elements, 4 of
+    4-byte ones), so a wave reads 1 KB of a row at a time: the 512^3 FP64 7-point
+    update measured 376 us against 534 us with one column per lane in a probe.
+    Where a tile lies inside the row and the array is 16-byte aligned with a row
+    pitch that is a multiple of CPL, a lane's columns are one 16-byte load;
+    otherwise one element at a time.  XGRID_STAGE3_COLS=1 selects one column per
+    lane, XGRID_STAGE3_F32=0 leaves 4-byte arrays unstaged."""
 
     NP, ROWS = 4, 4
     MAX_WINDOW = 8          # most planes held in registers per array
     MAX_LDS_GROUPS = 3      # most (array, plane offset) pairs staged in LDS
 
-    def __init__(self, refs, types):
+    def __init__(self, refs, types, sizes, esize: int = 8):
         self.refs = refs            # [(grid, level)]
         self.types = types          # device element type per ref index
+        self.sizes = sizes          # element bytes per ref index
+        self.esize = esize          # bytes of the statement's elements: arrays of this size are staged
         self.planes = {}            # j -> set of o0 read by staged loads
         self.xg = []                # [(j, o0)] with a column-neighbour load
         self.xside = {}             # (j, o0) -> set of +-1
         self.yg = []                # [(j, o0)] with a row-neighbour load
-        self.cpl = 1 if os.environ.get("XGRID_STAGE3_COLS", "2") == "1" else 2
+        # 16 bytes of a row per lane: 2 columns of 8-byte elements, 4 of 4-byte ones
+        self.cpl = 1 if os.environ.get("XGRID_STAGE3_COLS", "") == "1" else 16 // esize
         self.COLS = 64 * self.cpl   # columns of a tile
 
     def add(self, j, off) -> bool:
         o0, o1, o2 = off
-        if self._size(j) != 8:
-            # 4-byte arrays stay global loads: a wave's row segment is then two cache
-            # lines, and the plain loads beat the LDS copy and barrier (512^3 heat in
-            # FP32: 0.29 ms per step plain, 0.37 ms staged; FP64: 0.74 and 0.48 ms)
+        if self._size(j) != self.esize:
+            return False            # an array of another element size keeps global loads
+        if self.esize == 4 and os.environ.get("XGRID_STAGE3_F32", "1") == "0":
             return False
         if o1 == 0 and o2 == 0:
             self.planes.setdefault(j, set()).add(o0)
@@ -465,7 +471,7 @@ class _St3Plan:
         return sum(self.NP * rows * self.COLS * self._size(j) for j, _ in self.yg)
 
     def _size(self, j) -> int:
-        return 8 if self.types[j] in ("double", "int64_t", "uint64_t") else 4
+        return self.sizes[j]
 
     def members(self, expr: str) -> list:
         """st3_load: the plane registers, the edge lanes' outer columns and the
@@ -502,15 +508,17 @@ class _St3Plan:
         for j in sorted(self.planes):
             t = self.types[j]
             out.append(f"    const xg::gptr<const {t}> g{j} = xg::gp<const {t}>(R.v[{j}].ptr);")
-            if K == 2:
-                # pairs at 16-byte aligned addresses: a 16-byte aligned base, an even row
-                # pitch (then every plane pitch is even) and even columns
-                out.append(f"    if (full && ((uint64_t)R.v[{j}].ptr & 15) == 0 && (P1 & 1) == 0) {{")
+            if K * self._size(j) == 16:
+                # a lane's columns at a 16-byte aligned address: a 16-byte aligned base,
+                # a row pitch that is a multiple of K (then every plane pitch is) and
+                # column multiples of K
+                out.append(f"    if (full && ((uint64_t)R.v[{j}].ptr & 15) == 0 && (P1 % {K}) == 0) {{")
                 out.append("#pragma unroll")
                 out.append(f"      for (int u = 0; u < {self.window(j)}; ++u) {{")
                 out.append(f"        const xg::gptr<const {t}> rp = g{j} + (xg::st3_plane(p0 + u + {self.pmin(j)}, sh) * P0 + row * P1);")
-                out.append(f"        const xg::vec2<{t}> v = *(xg::gptr<const xg::vec2<{t}>>)(rp + cl);")
-                out.append(f"        st.q{j}[u][0] = v.x; st.q{j}[u][1] = v.y;")
+                out.append(f"        const xg::vecn<{t}, {K}> v = *(xg::gptr<const xg::vecn<{t}, {K}>>)(rp + cl);")
+                out.append("#pragma unroll")
+                out.append(f"        for (int k = 0; k < {K}; ++k) st.q{j}[u][k] = v[k];")
                 out.append("      }")
                 out.append("    } else {")
             else:
@@ -542,16 +550,16 @@ class _St3Plan:
             out.append(f"        for (int k = 0; k < {K}; ++k) L[(n * {R2} + wr + 1) * {C} + lane * {K} + k] = "
                        f"st.q{j}[n + {o0 - self.pmin(j)}][k];")
             # the halo rows above and below the tile: 2 x NP rows of C columns over 256 work-items
-            if K == 2:
-                # two columns per work-item, one 16-byte load, where aligned as above
-                out.append(f"      if (full && ((uint64_t)R.v[{j}].ptr & 15) == 0 && (P1 & 1) == 0) {{")
+            if K * self._size(j) == 16:
+                # K columns per work-item, one 16-byte load, where aligned as above
+                out.append(f"      if (full && ((uint64_t)R.v[{j}].ptr & 15) == 0 && (P1 % {K}) == 0) {{")
                 out.append("#pragma unroll")
-                out.append(f"        for (int h = 0; h < {2 * NP * C // 512}; ++h) {{")
-                out.append(f"          const int e = threadIdx.x + h * 256, cc = (e % {C // 2}) * 2, "
-                           f"pl = (e / {C // 2}) % {NP}, hr = e / {C // 2 * NP};")
+                out.append(f"        for (int h = 0; h < {2 * NP * 64 // 256}; ++h) {{")
+                out.append(f"          const int e = threadIdx.x + h * 256, cc = (e % 64) * {K}, "
+                           f"pl = (e / 64) % {NP}, hr = e / {64 * NP};")
                 out.append(f"          const int64_t gr = min(max(hr ? r0 + {self.ROWS} : r0 - 1, 0), sh[1] - 1);")
-                out.append(f"          *(xg::vec2<{t}>*)(L + (pl * {R2} + (hr ? {R2 - 1} : 0)) * {C} + cc) = "
-                           f"*(xg::gptr<const xg::vec2<{t}>>)(g{j} + (xg::st3_plane(p0 + pl + {o0}, sh) * P0 + gr * P1 + c0 + cc));")
+                out.append(f"          *(xg::vecn<{t}, {K}>*)(L + (pl * {R2} + (hr ? {R2 - 1} : 0)) * {C} + cc) = "
+                           f"*(xg::gptr<const xg::vecn<{t}, {K}>>)(g{j} + (xg::st3_plane(p0 + pl + {o0}, sh) * P0 + gr * P1 + c0 + cc));")
                 out.append("        }")
                 out.append("      } else {")
             else:
@@ -689,7 +697,11 @@ class Emitter:
         if os.environ.get("XGRID_STAGE3", "1") == "0":
             return None
         types = [device_type_name(grids[g].element) for g, _ in refs]
-        plan = _St3Plan(refs, types)
+        sizes = [element_nbytes(grids[g].element) for g, _ in refs]
+        esize = element_nbytes(st.gtype.element)
+        if esize not in (4, 8):
+            return None
+        plan = _St3Plan(refs, types, sizes, esize)
         for n in ir.walk(st.node.value):
             if isinstance(n, ir.Cell) and grids[n.grid.name].dimension == 3:
                 plan.add(refs.index((n.grid.name, n.level)), tuple(n.offsets))
