@@ -385,16 +385,16 @@ def leg_serving(args) -> dict:
     srv = Server("float", hbm_budget=budget, steps_per_tick=args.serve_tick,
                  max_cells=int(args.serve_max_mcells * 1e6) if args.serve_max_mcells else None)
     tuning = srv.ms.autotune(trace, limit=args.tune_limit) if args.tune_limit else {}
-    def progress(label):
+    def progress(label, total):
         def on_done(_, n):
-            if n % 25 == 0 or n == len(trace):
-                print(f"# serving {label}: {n} of {len(trace)} requests done", file=sys.stderr, flush=True)
+            if n % 25 == 0 or n == total:
+                print(f"# serving {label}: {n} of {total} requests done", file=sys.stderr, flush=True)
         return on_done
 
     def served_and_plain(tr, label):
-        got, summ = srv.run(tr, on_done=progress(f"batched{label}"))
+        got, summ = srv.run(tr, on_done=progress(f"batched{label}", len(tr)))
         summ.update(peak_resident_requests=srv.peak_active, peak_resident_bytes=srv.peak_used, ticks=srv.ticks)
-        ref, summ_plain = serve_plain(tr, "float", solver=srv.ms, on_done=progress(f"plain FIFO{label}"))
+        ref, summ_plain = serve_plain(tr, "float", solver=srv.ms, on_done=progress(f"plain FIFO{label}", len(tr)))
         same = all(np.array_equal(got[r].fields[c], ref[r].fields[c]) for r in got for c in "bpuv")
         return summ, summ_plain, same
 

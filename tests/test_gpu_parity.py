@@ -118,6 +118,33 @@ def test_heat_3d(prec, n, request):
     assert np.array_equal(d.now, R.heat_3d(u0, nu, dt, dx, steps, dt_np))
 
 
+@pytest.mark.parametrize("prec", ["float", "double"])
+def test_heat_3d_full_size(prec, request):
+    """The 3-D heat workload the bench times, at its own size (512^3, seed 0,
+    nu = 0.1, dt = 0.1 dx^2 / nu): the staged kernels (256-column tiles in FP32,
+    128 in FP64, 16-byte loads and stores) for 3 steps, bit for bit against the
+    numpy restatement of the reference's C."""
+    xg = request.getfixturevalue("double" if prec == "double" else "single")
+    from xgrid_amd import workloads as W
+    from xgrid_amd.backend import lower_kernel
+    n, steps, nu = 512, 3, 0.1
+    dt_np = np.float64 if prec == "double" else np.float32
+    u0, mask, dx = W.heat_3d_setup(n, dt_np, seed=0)
+    dt = 0.1 * dx * dx / nu
+    k = W.make_heat_3d()
+    low = lower_kernel(k)[0]
+    assert low.stmts[0].tile3_cols == (256 if prec == "float" else 128)
+    g = xg.Grid((n, n, n), float)
+    g.now[...] = u0
+    g.boundary[...] = mask
+    for _ in range(steps):
+        k(g, nu, dt, dx)
+    got = g.now
+    want = R.heat_3d(u0, nu, dt, dx, steps, dt_np)
+    assert got.dtype == want.dtype
+    assert np.array_equal(got.view(np.uint8), want.view(np.uint8))
+
+
 def test_convection_and_diffusion_2d(double):
     from xgrid_amd import workloads as W
     nx = 41
