@@ -122,9 +122,10 @@ def test_batched_diffusion_2d_equal_alone(double):
 
 @pytest.mark.parametrize("prec", ["double", "float"])
 def test_batched_heat_3d_equal_alone(prec, request):
-    """Batched 3-D statements (staged loads in FP64, plain loads in FP32) on
-    non-cubic grids, one narrower than a 64-column tile, equal the same
-    instances run alone."""
+    """Batched 3-D statements (staged loads: 128-column tiles in FP64, 256 in
+    FP32) on non-cubic grids, one narrower than a 64-column tile and one with two
+    full tiles of either width, equal the same instances run alone and the numpy
+    restatement of the reference's C."""
     xg = request.getfixturevalue("double" if prec == "double" else "single")
     from xgrid_amd import workloads as W
     from xgrid_amd.batch import Batch
@@ -133,7 +134,7 @@ def test_batched_heat_3d_equal_alone(prec, request):
     rng = np.random.default_rng(5)
     dt_np = np.float64 if prec == "double" else np.float32
     pairs = []
-    for shape in ((34, 34, 34), (20, 45, 150), (66, 9, 130), (20, 30, 40)):
+    for shape in ((34, 34, 34), (20, 45, 150), (66, 9, 130), (20, 30, 40), (7, 9, 600)):
         u0 = rng.random(shape).astype(dt_np)
         nu, dx = 0.1, 1.0 / (max(shape) - 1)
         dt = 0.1 * dx * dx / nu
@@ -144,12 +145,13 @@ def test_batched_heat_3d_equal_alone(prec, request):
             g.boundary[...] = W.box_walls(shape)
             gs.append(g)
         batch.add(gs[0], nu, dt, dx)
-        pairs.append((gs, (nu, dt, dx)))
+        pairs.append((gs, (nu, dt, dx), u0))
     batch.step(7)
-    for gs, a in pairs:
+    for gs, a, u0 in pairs:
         for _ in range(7):
             kp(gs[1], *a)
         _assert_same([gs[0]], [gs[1]])
+        assert np.array_equal(gs[0].now, R.heat_3d(u0, *a, 7, dt_np))
     batch.close()
 
 
