@@ -201,10 +201,11 @@ __device__ __forceinline__ void unlin(const int32_t* shape, int64_t c, int& i0, 
 
 /* This workgroup's tile of the box [lo, hi): the NC cells of this work-item,
  * whether each is inside the box, and the tile's extent clipped to the box.
- * CPL (3-D only): columns per work-item.  With CPL = 2 a tile is 128 columns
- * wide and work-item (r, c) holds columns 2c and 2c + 1 of each of its planes:
- * cell n is plane n / 2, column 2c + n % 2 (statements whose staged loads read
- * two columns per lane). */
+ * CPL (3-D only): columns per work-item.  A tile is 64 * CPL columns wide and
+ * work-item (r, c) holds columns CPL * c .. CPL * c + CPL - 1 of each of its
+ * planes: cell n is plane n / CPL, column CPL * c + n % CPL (statements whose
+ * staged loads read 16 bytes of a row per lane: CPL = 2 for 8-byte elements,
+ * 4 for 4-byte ones). */
 template <int D, int CPL = 1>
 struct TileCells {
   static constexpr int NC = Tile<D>::NC * (D == 3 ? CPL : 1);
@@ -218,7 +219,8 @@ struct TileCells {
     typedef Tile<D> TL;
     const int tid = threadIdx.x;
     int t0 = (int)tile, t1 = 0, t2 = 0;
-    /* 3-D column tiles start at multiples of 64 columns, not at lo[2]:
+    /* 3-D column tiles start at multiples of their width E2 (64 columns per
+     * column a lane holds), not at lo[2]:
      * with a row pitch that is a multiple of the 128-byte cache line, a wave's
      * row segment then covers whole lines instead of straddling one more, and
      * a line is fetched once instead of in parts by two tiles (512^3 FP32:
