@@ -5,6 +5,7 @@ import ctypes
 import dataclasses
 import random
 import re
+import typing
 
 import numpy as np
 import pytest
@@ -192,6 +193,38 @@ def test_3d_staged_neighbours_of_other_planes(double):
     run_pair(k, (du, dv), (hu, hv), steps=3)
     assert_bitexact(du, hu)
     assert_bitexact(dv, hv)
+
+
+@pytest.mark.parametrize("prec", ["float", "double"])
+def test_3d_staged_mixed_element_sizes(prec, request):
+    """A 3-D statement reading neighbours of a float grid and of an int grid:
+    with FP32 floats both arrays are 4 bytes and both are staged (the int array
+    through the same lane shifts and LDS copy); with FP64 floats only the 8-byte
+    array is, and the int array keeps global loads.  Bit for bit against the
+    oracle on rows with two full 256-column tiles and a partial one."""
+    xg = request.getfixturevalue("double" if prec == "double" else "single")
+    i3 = xgrid.grid[int, 3]
+
+    @xgrid.kernel()
+    def mixed(u: f3, v: f3, w: i3) -> None:
+        u[0, 0, 0] = (v[0, 0, 1] * 0.5 + v[0, -1, 0] + typing.cast(float, w[0, 0, -1] - w[0, 1, 0]) * 0.25 +
+                          v[1, 0, 0] * typing.cast(float, w[0, 0, 0]))
+
+    from xgrid_amd.backend import lower_kernel
+    src = lower_kernel(mixed)[0].device_src
+    st = src[src.index("struct xg_s0_t "):src.index("extern", src.index("struct xg_s0_t "))]
+    assert "ST3 = true" in st
+    if prec == "float":
+        assert "CPL = 4" in st and "int32_t q" in st          # the int array staged too
+    else:
+        assert "CPL = 2" in st and "int32_t q" not in st      # 4-byte array: global loads
+    rng = np.random.default_rng(31)
+    shape = (6, 9, 600)
+    du, hu = grid_pair(xg, shape, float, W.box_walls(shape), rng.random(shape))
+    dv, hv = grid_pair(xg, shape, float, None, rng.random(shape))
+    dw, hw = grid_pair(xg, shape, int, None, rng.integers(-50, 50, shape).astype(np.int32))
+    run_pair(mixed, (du, dv, dw), (hu, hv, hw), steps=2)
+    assert_bitexact(du, hu)
 
 
 def test_twophase_store_offset(double):
