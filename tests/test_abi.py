@@ -96,6 +96,7 @@ def test_generated_kernels_compile_for_gfx950(prec, request, tmp_path):
         op = make()
         low, _ = lower_kernel(op)
         assert os.path.getsize(compile_device(low.device_src, str(tmp_path))) > 1000
+        assert os.path.getsize(compile_device(low.batch_src, str(tmp_path))) > 1000
         stub = compile_host(low.stub_src, str(tmp_path), ["gcc"])
         assert os.path.exists(stub)
         # batches call the stub with every argument packed in one struct: ctypes must agree with C
@@ -106,18 +107,26 @@ def test_generated_kernels_compile_for_gfx950(prec, request, tmp_path):
 
 
 def test_generated_device_source_has_batched_variants(double):
+    """Every launch has a batched variant, in the second source (compiled when a
+    batch is made); the plain source defines none of them, and both sources hold
+    every plain launch's statement structs."""
+    import re
     from xgrid_amd import workloads as W
     from xgrid_amd.backend import lower_kernel
     low, _ = lower_kernel(W.make_cavity())
-    src = low.device_src
+    src, bsrc = low.device_src, low.batch_src
     for m in low.stmts:
-        assert f"{m.kernel}_b(" in src
+        assert f" {m.kernel}(" in src and f"{m.kernel}_b(" in bsrc and f"{m.kernel}_b(" not in src
         if m.kernel_pp:
-            assert f"{m.kernel_pp}b(" in src
+            assert f"{m.kernel_pp}b(" in bsrc
+        assert f"struct xg_s{m.stmt.sid}_t " in src and f"struct xg_s{m.stmt.sid}_t " in bsrc
     for g in low.groups:
-        assert f"{g.kernel}_b(" in src
+        assert f"{g.kernel}_b(" in bsrc
     for L in low.loops:
-        assert f"{L.kernel}_b(" in src and f"{L.kernel}_chkb(" in src
+        assert f"{L.kernel}_b(" in bsrc and f"{L.kernel}_chkb(" in bsrc
+    defined = lambda s: set(re.findall(r'extern "C" __global__[^\n]*?\b(xg_\w+)\(', s))   # noqa: E731
+    assert not defined(src) & defined(bsrc)
+    assert all(re.search(r"_(b|chkb|ppb)$", n) for n in defined(bsrc))
 
 
 def test_rtc_compile_error_is_reported(tmp_path):

@@ -362,38 +362,35 @@ def test_staged_lane_shifts_run_on_every_lane(double):
         with xgrid.boundary(4):
             u[0, 0, 0] = u[0, 0, 1][0] + v[0, -1, 1] - v[0, 0, -1]
 
-    for op in (W.make_heat_3d(), mixed):
-        src = lower_kernel(op)[0].device_src
-        shifts = [ln.strip() for ln in src.splitlines() if "lane_shift<" in ln]
-        assert shifts, "no staged statement"
-        assert all(re.fullmatch(r"const \w+ [ab] = xg::lane_shift<-?1>\(c[fl]\);", ln) for ln in shifts), shifts
+    def assembly(text):
         with tempfile.TemporaryDirectory() as d:
             p = os.path.join(d, "k.hip")
             with open(p, "w") as f:
-                f.write(src)
+                f.write(text)
             r = subprocess.run([hipcc, "-x", "hip", "--cuda-device-only", "-S", *compilers.device_options(),
                                 "-I", compilers.CSRC, p, "-o", os.path.join(d, "k.s")],
                                capture_output=True, text=True, timeout=300)
             assert r.returncode == 0, r.stderr[-2000:]
-            asm = open(os.path.join(d, "k.s")).read()
-        found = _dpp_in_masked_regions(asm)
-        assert found, "no staged kernel with lane shifts"
-        assert all(bad == 0 for _, bad in found.values()), found
+            return open(os.path.join(d, "k.s")).read()
+
+    for op in (W.make_heat_3d(), mixed):
+        low = lower_kernel(op)[0]
+        src = low.device_src
+        shifts = [ln.strip() for ln in src.splitlines() if "lane_shift<" in ln]
+        assert shifts, "no staged statement"
+        assert all(re.fullmatch(r"const \w+ [ab] = xg::lane_shift<-?1>\(c[fl]\);", ln) for ln in shifts), shifts
+        found = _dpp_in_masked_regions(assembly(src))
+        found_b = _dpp_in_masked_regions(assembly(low.batch_src))
+        assert found and found_b, "no staged kernel with lane shifts"
+        assert any(k.endswith("_b") for k in found_b)
+        assert all(bad == 0 for _, bad in list(found.values()) + list(found_b.values())), (found, found_b)
         if op is not mixed:
             continue
         # the planted mistake: a shift evaluated only where the conditional selects it
         planted = re.sub(r"const (\w+) a = (xg::lane_shift<1>\(cf\));\n(\s*)(st\.xp\d+) = (st\.lane == 63 \? [^:]+ : )a;",
                          r"\3\4 = \5\2;", src, count=1)
         assert planted != src
-        with tempfile.TemporaryDirectory() as d:
-            p = os.path.join(d, "k.hip")
-            with open(p, "w") as f:
-                f.write(planted)
-            r = subprocess.run([hipcc, "-x", "hip", "--cuda-device-only", "-S", *compilers.device_options(),
-                                "-I", compilers.CSRC, p, "-o", os.path.join(d, "k.s")],
-                               capture_output=True, text=True, timeout=300)
-            assert r.returncode == 0, r.stderr[-2000:]
-            bad = _dpp_in_masked_regions(open(os.path.join(d, "k.s")).read())
+        bad = _dpp_in_masked_regions(assembly(planted))
         assert any(b > 0 for _, b in bad.values()), bad
 
 

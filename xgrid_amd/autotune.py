@@ -48,7 +48,7 @@ def _key(op, precision: str, sizes: list, candidates: list) -> str:
     buf = ctypes.create_string_buffer(256)
     lib.xg_device_name(buf, 256, None, None)
     h = hashlib.sha256()
-    for part in (low.device_src, precision, buf.value.decode(), json.dumps(sorted(sizes)), json.dumps(candidates),
+    for part in (low.device_src, low.batch_src, precision, buf.value.decode(), json.dumps(sorted(sizes)), json.dumps(candidates),
                  get_config().overstep):
         h.update(part.encode())
     return h.hexdigest()[:32]

@@ -51,15 +51,17 @@ def packed_args_type(arg_types):
     return type("xg_args", (ctypes.Structure,), {"_fields_": fields_})
 
 
-def _release(lib, plan, module):
+def _release(lib, plan, modules):
     """Destroy a kernel's own launch plan (its graph execs, events, flag) and unload
-    its code object.  Plans made by new_plan() for batch instances belong to their
+    its code objects (`modules`: a list the kernel appends the batched module to
+    when it loads it).  Plans made by new_plan() for batch instances belong to their
     batch, which holds the kernel and destroys them first."""
     try:
         if plan:
             lib.xg_kernel_destroy(plan)
-        if module:
-            lib.xg_module_unload(module)
+        for module in modules:
+            if module:
+                lib.xg_module_unload(module)
     except Exception:
         pass
 
@@ -86,17 +88,12 @@ class HipKernel:
         mod = c_void_p()
         _lib.check(self.lib.xg_module_load(self.code_object.encode(), ctypes.byref(mod)), "xg_module_load")
         self.module = mod.value
+        self.module_b = None             # the batched variants: loaded by ensure_batch()
+        self._modules = [self.module]
         self.kernel = None
-        self._finalizer = weakref.finalize(self, _release, self.lib, None, self.module)
+        self._finalizer = weakref.finalize(self, _release, self.lib, None, self._modules)
         self._finalizer.atexit = False
-
-        def fn(name):
-            if name is None:
-                return None
-            out = c_void_p()
-            _lib.check(self.lib.xg_module_function(self.module, name.encode(), ctypes.byref(out)),
-                       f"kernel {name}")
-            return out.value
+        fn = self._function_of(self.module)
 
         grid_index = {n: k for k, (_, n, _) in enumerate(low.grid_args)}
         self.grid_index = grid_index
@@ -134,9 +131,6 @@ class HipKernel:
             d.fn_main = fn(m.kernel)
             d.fn_pingpong = fn(m.kernel_pp)
             d.fn_phase2 = fn(m.phase2)
-            d.fn_main_b = fn(m.kernel + "_b")
-            d.fn_pingpong_b = fn(m.kernel_pp + "b") if m.kernel_pp else None
-            d.fn_phase2_b = fn(m.phase2 + "_b")
             d.fn_pair = fn(m.pair) if m.pair else None
             d.pair_fwd = int(m.pair_fwd)
             d.tile3_cols = m.tile3_cols
@@ -145,7 +139,6 @@ class HipKernel:
             groups[i].first = g.first
             groups[i].count = g.count
             groups[i].fn = fn(g.kernel)
-            groups[i].fn_b = fn(g.kernel + "_b")
         # loop plans: one per blockable loop (ids as in the stub), then the small-grid
         # tiles of those that have one
         alts = [L.alt for L in low.loops if L.alt is not None]
@@ -163,8 +156,6 @@ class HipKernel:
             ld.threads = L.threads
             ld.fn = fn(L.kernel)
             ld.fn_check = fn(L.kernel + "_chk")
-            ld.fn_b = fn(L.kernel + "_b") if L.batched else None
-            ld.fn_check_b = fn(L.kernel + "_chkb") if L.batched else None
             ld.alt = -1
             if i < len(low.loops) and L.alt is not None:
                 ld.alt = len(low.loops) + nalt
@@ -174,7 +165,7 @@ class HipKernel:
         self._descs, self._groups, self._loops = descs, groups, loops
         self.kernel = self.new_plan()
         self._finalizer.detach()
-        self._finalizer = weakref.finalize(self, _release, self.lib, self.kernel, self.module)
+        self._finalizer = weakref.finalize(self, _release, self.lib, self.kernel, self._modules)
         self._finalizer.atexit = False
         self.set_options()
         self.api = self.lib.xg_api_table()
@@ -192,12 +183,48 @@ class HipKernel:
         if ctypes.sizeof(self.Args) != self.stub.xg_args_size():
             raise KernelError("packed argument layout differs between the stub and ctypes")
 
+    def _function_of(self, module):
+        def fn(name):
+            if name is None:
+                return None
+            out = c_void_p()
+            _lib.check(self.lib.xg_module_function(module, name.encode(), ctypes.byref(out)), f"kernel {name}")
+            return out.value
+        return fn
+
+    def ensure_batch(self) -> None:
+        """Compile (or find in the cache) and load the batched variants of this kernel's
+        launches, and enter them in the descriptors: plans made by new_plan() after this
+        can run in a batch.  Plain calls never need them, so they are built only when a
+        batch of this kernel is made."""
+        if self.module_b is not None:
+            return
+        cfg = self.cfg
+        path = compile_device(self.lowered.batch_src, cfg.cacheroot, cfg.fp_contract)
+        mod = c_void_p()
+        _lib.check(self.lib.xg_module_load(path.encode(), ctypes.byref(mod)), "xg_module_load")
+        self.module_b = mod.value
+        self._modules.append(self.module_b)
+        fn = self._function_of(self.module_b)
+        low = self.lowered
+        for i, m in enumerate(low.stmts):
+            d = self._descs[i]
+            d.fn_main_b = fn(m.kernel + "_b")
+            d.fn_pingpong_b = fn(m.kernel_pp + "b") if m.kernel_pp else None
+            d.fn_phase2_b = fn(m.phase2 + "_b")
+        for i, g in enumerate(low.groups):
+            self._groups[i].fn_b = fn(g.kernel + "_b")
+        for i, L in enumerate(list(low.loops) + [L.alt for L in low.loops if L.alt is not None]):
+            self._loops[i].fn_b = fn(L.kernel + "_b") if L.batched else None
+            self._loops[i].fn_check_b = fn(L.kernel + "_chkb") if L.batched else None
+
     def close(self) -> None:
-        """Destroy the launch plan and unload the code object now.  Batch plans made
+        """Destroy the launch plan and unload the code objects now.  Batch plans made
         with new_plan() must have been destroyed already (Batch.close)."""
         self._finalizer()
         self.kernel = None
         self.module = None
+        self.module_b = None
 
     def new_plan(self) -> int:
         """A fresh launch plan (xg_kernel) over this kernel's code: one per batch instance."""

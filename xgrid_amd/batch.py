@@ -55,6 +55,7 @@ class Batch:
             native = op.native
         self.op = op
         self.native = native
+        native.ensure_batch()            # the batched kernels, before any instance's plan is made
         self.depth = native.depth
         self.lib = _lib.load()
         h = ctypes.c_void_p()

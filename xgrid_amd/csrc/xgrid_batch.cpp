@@ -746,6 +746,12 @@ int xg_batch_add(xg_batch* b, xg_kernel* k, const void* args, int64_t arg_bytes,
                  int32_t nticks, int64_t* id) {
   if (!b || !k || (!args && arg_bytes > 0) || arg_bytes < 0 || nticks < 0 || (nticks > 0 && !tick_grids))
     return fail(XG_ERR_ARG, "invalid batch instance");
+  // the batched variants come from a second code object that the host loads on
+  // demand: a plan made before it was loaded cannot run in a batch
+  for (auto& sp : k->stmts)
+    if (!sp.d.fn_main_b) return fail(XG_ERR_STATE, "the kernel's batched variants are not loaded");
+  for (auto& g : k->groups)
+    if (!g.fn_b) return fail(XG_ERR_STATE, "the kernel's batched variants are not loaded");
   for (auto& I : b->inst) {
     if (I.k == k) return fail(XG_ERR_ARG, "the kernel plan already belongs to this batch");
     for (auto* G : I.ticks)

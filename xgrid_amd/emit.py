@@ -22,6 +22,7 @@ otherwise.
 from __future__ import annotations
 
 import os
+import re
 from dataclasses import dataclass, field
 
 from . import ir
@@ -131,7 +132,7 @@ class LoopMeta:
 
 @dataclass
 class Lowered:
-    device_src: str
+    device_src: str               # the kernels plain calls launch
     stub_src: str
     stmts: list
     groups: list
@@ -140,6 +141,39 @@ class Lowered:
     graphs_ok: bool
     loops: list = field(default_factory=list)
     block_shape: tuple | None = None   # shape of the blocked loops (None: the kernel has none)
+    batch_src: str = ""           # the batched variants (many instances per launch), compiled on demand
+
+
+_BATCHED_KERNEL = re.compile(r"_(b|chkb|ppb)$")
+
+
+def split_batched(src: str) -> tuple:
+    """(plain source, batched source) of one emitted module.  Both keep every line
+    outside the kernel definitions (the statement structs, templates and device
+    functions: templates cost nothing until a kernel instantiates them); each kernel
+    definition goes to one of them, by its name (batched variants end in _b, _chkb or
+    _ppb).  Plain calls then compile only what they launch: the batched kernels were
+    about 40 % of the compile time of the 2049^2 cavity module."""
+    lines = src.split("\n")
+    plain, batch = [], []
+    i = 0
+    while i < len(lines):
+        line = lines[i]
+        if not line.startswith('extern "C" __global__'):
+            plain.append(line)
+            batch.append(line)
+            i += 1
+            continue
+        name = re.search(r"\b(xg_\w+)\(", line).group(1)
+        block = [line]
+        depth = line.count("{") - line.count("}")
+        i += 1
+        while depth > 0:
+            block.append(lines[i])
+            depth += lines[i].count("{") - lines[i].count("}")
+            i += 1
+        (batch if _BATCHED_KERNEL.search(name) else plain).extend(block)
+    return "\n".join(plain), "\n".join(batch)
 
 
 class _ExprC:
@@ -1673,10 +1707,11 @@ class Emitter:
         loops = self._blockable_loops(stmt_meta)
         graphs_ok = not any(isinstance(n, ir.Inline) for n in ir.walk_all(self.func.body)) and \
             not any(op.mode == "external" and not is_device_math(op) for op in self.functions)
-        return Lowered(self.device_source(stmt_meta, groups, loops), self.stub_source(stmt_meta, loops), stmt_meta,
+        plain_src, batch_src = split_batched(self.device_source(stmt_meta, groups, loops))
+        return Lowered(plain_src, self.stub_source(stmt_meta, loops), stmt_meta,
                        groups, list(self.func.grid_args),
                        [(i, n, t) for i, (n, t) in enumerate(self.func.args) if not isinstance(t, GridType)],
-                       graphs_ok, loops, self.used_shape)
+                       graphs_ok, loops, self.used_shape, batch_src)
 
     def _hoists(self, st: StencilStmt) -> list:
         """The statement's maximal launch-uniform floating-point subexpressions that

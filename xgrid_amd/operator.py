@@ -107,10 +107,12 @@ class Operator:
 
     @property
     def src(self) -> str:
-        """Generated device source and host stub of a kernel."""
+        """Generated device source (plain launches, then the batched variants) and host
+        stub of a kernel."""
         from .backend import lower_kernel
         low, _ = lower_kernel(self)
-        return low.device_src + "\n/* ---- host stub ---- */\n" + low.stub_src
+        return (low.device_src + "\n/* ---- batched variants ---- */\n" + low.batch_src +
+                "\n/* ---- host stub ---- */\n" + low.stub_src)
 
     def signature_args(self) -> list:
         return list(self.ir.args)
