@@ -356,17 +356,38 @@ __device__ __forceinline__ void box_cells(const xg_launch_hdr& H, const typename
     }
     if constexpr (D == 3 && S::CPL * sizeof(T) == 16) {
       if (pairs) {
+        /* A lane whose CPL cells are only partly stored (the class edge, a hole of a
+         * masked class) merges its computed cells into the current values of the
+         * others and stores all 16 bytes, so every lane of the wave takes the same
+         * single store.  No other work-item of the launch writes those cells: they
+         * are outside the statement's class, or outside the columns another tile
+         * covers (tiles start at multiples of their width), and staged stores run
+         * only unshifted and in place. */
         constexpr int K = S::CPL;
+        const int col = tc.col0 + (int)(threadIdx.x & 63) * K;
+        const bool row_fits = col + K <= H.out.shape[2];
 #pragma unroll
         for (int n = 0; n < NC; n += K) {
-          bool all = true;
+          bool all = true, any = false;
 #pragma unroll
-          for (int k = 0; k < K; ++k) all = all && tc.in[n + k];
-          if (all) {
+          for (int k = 0; k < K; ++k) {
+            all = all && tc.in[n + k];
+            any = any || tc.in[n + k];
+          }
+          if (!any) continue;
+          if (row_fits) {
+            const gptr<vecn<T, K>> dst =
+                (gptr<vecn<T, K>>)(gp<T>(H.out.ptr) + lin<3>(H.out.shape, tc.c[n][0], tc.c[n][1], col));
             vecn<T, K> w;
 #pragma unroll
             for (int k = 0; k < K; ++k) w[k] = v[n + k];
-            *(gptr<vecn<T, K>>)(gp<T>(H.out.ptr) + lin<3>(H.out.shape, tc.c[n][0], tc.c[n][1], tc.c[n][2])) = w;
+            if (!all) {
+              const vecn<T, K> old = *dst;
+#pragma unroll
+              for (int k = 0; k < K; ++k)
+                if (!tc.in[n + k]) w[k] = old[k];
+            }
+            *dst = w;
           } else {
 #pragma unroll
             for (int k = 0; k < K; ++k)
