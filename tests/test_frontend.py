@@ -249,12 +249,14 @@ def test_staged_3d_plan(double, monkeypatch):
 def _dpp_in_masked_regions(asm: str) -> dict:
     """Per kernel of an AMD assembly listing: (DPP moves, DPP moves executed
     inside an exec-masked region).  The kernel is split into basic blocks
-    (labels, and the ends of branches); the nesting depth of exec-masked regions
-    (+1 at s_and_saveexec, -1 at the s_or_b64 exec that ends a region) flows
-    along the branch and fall-through edges from depth 0 at entry, the largest
-    depth winning where paths join.  Following edges rather than layout order
-    matters: the compiler lays a region's blocks out before or after the block
-    that opens it."""
+    (labels, and the ends of branches).  The open exec-masked regions are a stack
+    of the registers their s_and_saveexec saved the mask into; an
+    `s_or_b64 exec, exec, sN` closes every region opened since sN was saved (the
+    compiler closes an inner region and the one around it with one restore of
+    the outer mask).  The stack flows along the branch and fall-through edges
+    from empty at entry, the deeper one winning where paths join.  Following
+    edges rather than layout order matters: the compiler lays a region's blocks
+    out before or after the block that opens it."""
     import re
     kernels, cur = {}, None
     for line in asm.splitlines():
@@ -271,6 +273,15 @@ def _dpp_in_masked_regions(asm: str) -> dict:
             continue
         if t:
             kernels[cur].append(t)
+
+    def step(stack, t):
+        if t.startswith("s_and_saveexec"):
+            return stack + (t.split()[1].rstrip(","),)
+        if t.startswith("s_or_b64 exec, exec,"):
+            reg = t.split(",")[-1].strip()
+            return stack[:stack.index(reg)] if reg in stack else stack[:-1]
+        return stack
+
     out = {}
     for name, lines in kernels.items():
         blocks, labels, b = [], {}, []
@@ -290,7 +301,8 @@ def _dpp_in_masked_regions(asm: str) -> dict:
                 b = []
         if b:
             blocks.append(b)
-        # labels map to the index of the block that starts after them
+        if not blocks:
+            continue
         succ = []
         for i, blk in enumerate(blocks):
             last = blk[-1] if blk else ""
@@ -304,34 +316,26 @@ def _dpp_in_masked_regions(asm: str) -> dict:
                 succ.append([])
             else:
                 succ.append(nxt)
-        depth_in = [None] * len(blocks)
-        if not blocks:
-            continue
-        depth_in[0] = 0
+        stack_in = [None] * len(blocks)
+        stack_in[0] = ()
         work = [0]
         while work:
             i = work.pop()
-            d = depth_in[i]
+            st = stack_in[i]
             for t in blocks[i]:
-                if t.startswith("s_and_saveexec"):
-                    d += 1
-                elif t.startswith("s_or_b64 exec, exec,"):
-                    d = max(0, d - 1)
+                st = step(st, t)
             for j in succ[i]:
-                if depth_in[j] is None or d > depth_in[j]:
-                    depth_in[j] = min(d, 64)
+                if stack_in[j] is None or (len(st) > len(stack_in[j]) and len(st) <= 64):
+                    stack_in[j] = st
                     work.append(j)
         n = bad = 0
         for i, blk in enumerate(blocks):
-            d = depth_in[i] if depth_in[i] is not None else 0
+            st = stack_in[i] if stack_in[i] is not None else ()
             for t in blk:
-                if t.startswith("s_and_saveexec"):
-                    d += 1
-                elif t.startswith("s_or_b64 exec, exec,"):
-                    d = max(0, d - 1)
+                st = step(st, t)
                 if "_dpp" in t:
                     n += 1
-                    bad += d > 0
+                    bad += len(st) > 0
         if n:
             out[name] = (n, bad)
     return out
