@@ -341,6 +341,67 @@ def _dpp_in_masked_regions(asm: str) -> dict:
     return out
 
 
+def _hipcc():
+    import shutil
+    return shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+
+
+def _need_hipcc():
+    if not os.path.exists(_hipcc()):
+        pytest.skip("hipcc not available")
+
+
+def _assembly(text):
+    """gfx950 assembly of a generated device source, built with the run-time options."""
+    import subprocess
+    import tempfile
+    from xgrid_amd import compilers
+    with tempfile.TemporaryDirectory() as d:
+        p = os.path.join(d, "k.hip")
+        with open(p, "w") as f:
+            f.write(text)
+        r = subprocess.run([_hipcc(), "-x", "hip", "--cuda-device-only", "-S", *compilers.device_options(),
+                            "-I", compilers.CSRC, p, "-o", os.path.join(d, "k.s")],
+                           capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0, r.stderr[-2000:]
+        return open(os.path.join(d, "k.s")).read()
+
+
+def _kernel_body(asm, name):
+    import re
+    m = re.search(rf"^{name}:[^\n]*\n(.*?)^\s*s_endpgm", asm, re.S | re.M)
+    assert m, name
+    return m.group(1)
+
+
+def test_nontemporal_stores_only_in_3d_box_kernels(double):
+    """An in-place statement over a grid larger than the last-level cache is
+    launched with XG_F_STREAM, and its 3-D box tiles then store with the
+    non-temporal hint.  The two store forms are separate template instances:
+    written as a run-time branch, the compiler merges them into one plain store
+    and the hint is lost.  So a 3-D kernel's code holds both forms (16-byte
+    staged stores included), and a 2-D kernel's only plain stores."""
+    import re
+    _need_hipcc()
+    from xgrid_amd import workloads as W
+    from xgrid_amd.backend import lower_kernel
+    f2 = xgrid.grid[float, 2]
+
+    @xgrid.kernel()
+    def smooth(u: f2, v: f2) -> None:
+        u[0, 0] = (v[0, 1] + v[0, -1] + v[1, 0] + v[-1, 0]) * 0.25
+
+    def stores(body):
+        forms = re.findall(r"^\s*(global_store_\w+)\s.*?(\bnt)?\s*$", body, re.M)
+        return {(op, bool(nt)) for op, nt in forms}
+
+    heat = stores(_kernel_body(_assembly(lower_kernel(W.make_heat_3d())[0].device_src), "xg_k0"))
+    assert ("global_store_dwordx4", True) in heat and ("global_store_dwordx4", False) in heat, heat
+    assert ("global_store_dwordx2", True) in heat and ("global_store_dwordx2", False) in heat, heat
+    flat = stores(_kernel_body(_assembly(lower_kernel(smooth)[0].device_src), "xg_k0"))
+    assert flat and not any(nt for _, nt in flat), flat
+
+
 def test_staged_lane_shifts_run_on_every_lane(double):
     """A lane shift reads another lane's register; run by a subset of lanes it
     reads 0 from the disabled ones.  In the generated source every shift is a
@@ -350,13 +411,8 @@ def test_staged_lane_shifts_run_on_every_lane(double):
     reached inside an exec-masked region.  The check itself is pinned by a
     planted shift inside the edge lane's conditional, which it must flag."""
     import re
-    import shutil
-    import subprocess
-    import tempfile
-    hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
-    if not os.path.exists(hipcc):
-        pytest.skip("hipcc not available")
-    from xgrid_amd import compilers, workloads as W
+    _need_hipcc()
+    from xgrid_amd import workloads as W
     from xgrid_amd.backend import lower_kernel
     f3 = xgrid.grid[float, 3]
 
@@ -366,16 +422,7 @@ def test_staged_lane_shifts_run_on_every_lane(double):
         with xgrid.boundary(4):
             u[0, 0, 0] = u[0, 0, 1][0] + v[0, -1, 1] - v[0, 0, -1]
 
-    def assembly(text):
-        with tempfile.TemporaryDirectory() as d:
-            p = os.path.join(d, "k.hip")
-            with open(p, "w") as f:
-                f.write(text)
-            r = subprocess.run([hipcc, "-x", "hip", "--cuda-device-only", "-S", *compilers.device_options(),
-                                "-I", compilers.CSRC, p, "-o", os.path.join(d, "k.s")],
-                               capture_output=True, text=True, timeout=300)
-            assert r.returncode == 0, r.stderr[-2000:]
-            return open(os.path.join(d, "k.s")).read()
+    assembly = _assembly
 
     for op in (W.make_heat_3d(), mixed):
         low = lower_kernel(op)[0]

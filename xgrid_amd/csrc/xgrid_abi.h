@@ -39,6 +39,7 @@ typedef __hip_internal::uint64_t uint64_t;
 #define XG_F_PINGPONG 1  /* iterate the whole grid; class cells computed, others copied from src */
 #define XG_F_TO_TMP 2    /* phase 1 of a two-phase statement: value -> src[cell] (no store offset) */
 #define XG_F_UNIFORM 4   /* every loaded view has the store grid's rank and shape */
+#define XG_F_STREAM 8    /* in place, store grid above the last-level cache: 3-D box tiles store non-temporally */
 
 typedef struct xg_gview {
   void* ptr;                 /* element 0 of one time level */

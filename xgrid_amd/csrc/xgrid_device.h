@@ -145,20 +145,30 @@ __device__ __forceinline__ T ld(const xg_gview& g, const int32_t* sh, int64_t xc
   return gp<const T>(g.ptr)[k];
 }
 
-/* Store one cell; a store outside the array is dropped. */
-template <class T, int D, int OV>
-__device__ __forceinline__ void st(const xg_gview& g, int i0, int i1, int i2, const T& v) {
-  const int64_t k = index<D, OV>(g, i0, i1, i2);
-  if (k >= 0 && k < cells<D>(g.shape)) gp<T>(g.ptr)[k] = v;
+/* Write v at dst; NT: non-temporally, so the line is evicted first and a grid
+ * larger than the cache does not push out the cells the launch still reads.
+ * The choice is a template argument: the compiler merges a run-time branch
+ * between the two forms into one plain store. */
+template <class V, bool NT>
+__device__ __forceinline__ void put(gptr<V> dst, const V& v) {
+  if constexpr (NT) __builtin_nontemporal_store(v, dst);
+  else *dst = v;
 }
 
-template <class S, int OV>
+/* Store one cell; a store outside the array is dropped. */
+template <class T, int D, int OV, bool NT = false>
+__device__ __forceinline__ void st(const xg_gview& g, int i0, int i1, int i2, const T& v) {
+  const int64_t k = index<D, OV>(g, i0, i1, i2);
+  if (k >= 0 && k < cells<D>(g.shape)) put<T, NT>(gp<T>(g.ptr) + k, v);
+}
+
+template <class S, int OV, bool NT = false>
 __device__ __forceinline__ void store(const xg_launch_hdr& H, int i0, int i1, int i2, const typename S::T& v) {
   typedef typename S::T T;
   if (H.dom.flags & XG_F_TO_TMP) {
     gp<T>(H.src.ptr)[lin<S::D>(H.out.shape, i0, i1, i2)] = v;
   } else {
-    st<T, S::D, OV>(H.out, i0 + S::SO0, i1 + S::SO1, i2 + S::SO2, v);
+    st<T, S::D, OV, NT>(H.out, i0 + S::SO0, i1 + S::SO1, i2 + S::SO2, v);
   }
 }
 
@@ -304,7 +314,7 @@ __device__ __forceinline__ bool tile_safe(const xg_launch_hdr& H, const int32_t*
   return ok;
 }
 
-template <class S, int OV, bool MASKED>
+template <class S, int OV, bool MASKED, bool NT = false>
 __device__ __forceinline__ void box_cells(const xg_launch_hdr& H, const typename S::Refs& R,
                                           const typename S::Params& P, uint32_t tile, unsigned char* st3_lds) {
   constexpr int D = S::D;
@@ -387,11 +397,11 @@ __device__ __forceinline__ void box_cells(const xg_launch_hdr& H, const typename
               for (int k = 0; k < K; ++k)
                 if (!tc.in[n + k]) w[k] = old[k];
             }
-            *dst = w;
+            put<vecn<T, K>, NT>(dst, w);
           } else {
 #pragma unroll
             for (int k = 0; k < K; ++k)
-              if (tc.in[n + k]) store<S, OV>(H, tc.c[n + k][0], tc.c[n + k][1], tc.c[n + k][2], v[n + k]);
+              if (tc.in[n + k]) store<S, OV, NT>(H, tc.c[n + k][0], tc.c[n + k][1], tc.c[n + k][2], v[n + k]);
           }
         }
         continue;
@@ -399,7 +409,7 @@ __device__ __forceinline__ void box_cells(const xg_launch_hdr& H, const typename
     }
 #pragma unroll
     for (int n = 0; n < NC; ++n)
-      if (tc.in[n]) store<S, OV>(H, tc.c[n][0], tc.c[n][1], tc.c[n][2], v[n]);
+      if (tc.in[n]) store<S, OV, NT>(H, tc.c[n][0], tc.c[n][1], tc.c[n][2], v[n]);
   }
 }
 
@@ -418,10 +428,16 @@ __device__ void run_main(const xg_launch_hdr& H, const typename S::Refs& R, cons
       apply<S, OV>(H, R, P, i0, i1, i2);
     }
   } else if constexpr (D == 3 && S::ST3) {
-    /* one LDS copy for both box variants (staged 3-D loads) */
+    /* one LDS copy for the box variants (staged 3-D loads) */
     __shared__ __attribute__((aligned(16))) unsigned char st3_lds[S::ST3_LDS];
     if (dom.kind == XG_DOM_MASKED) box_cells<S, OV, true>(H, R, P, tile, st3_lds);
+    else if (dom.flags & XG_F_STREAM) box_cells<S, OV, false, true>(H, R, P, tile, st3_lds);
     else box_cells<S, OV, false>(H, R, P, tile, st3_lds);
+  } else if constexpr (D == 3) {
+    /* 3-D box tiles of a grid larger than the cache store non-temporally */
+    if (dom.kind == XG_DOM_MASKED) box_cells<S, OV, true>(H, R, P, tile, nullptr);
+    else if (dom.flags & XG_F_STREAM) box_cells<S, OV, false, true>(H, R, P, tile, nullptr);
+    else box_cells<S, OV, false>(H, R, P, tile, nullptr);
   } else if (dom.kind == XG_DOM_MASKED) {
     box_cells<S, OV, true>(H, R, P, tile, nullptr);
   } else {

@@ -16,6 +16,10 @@ constexpr int kThreads = 256;
 constexpr int kGroupThreads = 1024;
 constexpr int64_t kFuseMaxCells = 4096;  // default of g_dev.fuse_max_cells (XGRID_FUSE_MAX_CELLS)
 constexpr int64_t kGraphMinLaunches = 4;    // a call with fewer launches runs them directly (cheaper)
+// Store grids larger than the 256 MB last-level cache (Infinity Cache) take non-temporal
+// stores: a later launch cannot find their cells in cache anyway, and the hint
+// leaves L2 to the cells the launch still reads (halo rows and planes).
+constexpr size_t kStreamBytes = (size_t)256 << 20;
 
 // kRecord: the stub's statements are recorded for a batched launch instead of run
 enum CallMode { kDirect = 0, kCapture = 1, kReplay = 2, kRecord = 3 };
@@ -108,6 +112,7 @@ int effective_mode(xg_kernel* k, StmtPlan& sp);
 void fill_dom(xg_dom& dom, const ClassPlan& p, const xg_grid* G, int32_t klass);
 void fill_view(xg_gview& v, const xg_grid* G, void* ptr);
 bool uniform_refs(xg_kernel* k, const xg_stmt_desc& d);
+int32_t stream_flag(const xg_grid* G);
 int fill_refs(xg_kernel* k, const xg_stmt_desc& d, std::vector<xg_gview>& refs);
 // Workgroups over the box [lo, hi) for a statement's tile; cols3 = columns of a 3-D tile.
 int64_t tiles_for(int D, const int32_t* lo, const int32_t* hi, int cols3 = 64);

@@ -166,6 +166,8 @@ void fill_view(xg_gview& v, const xg_grid* G, void* ptr) {
   for (int d = 0; d < 3; ++d) v.shape[d] = d < G->dims ? G->shape[d] : 1;
 }
 
+int32_t stream_flag(const xg_grid* G) { return G->bytes() > kStreamBytes ? XG_F_STREAM : 0; }
+
 // Does every array the statement loads have the store grid's rank and shape?
 bool uniform_refs(xg_kernel* k, const xg_stmt_desc& d) {
   const xg_grid* G = k->grids[(size_t)d.store_arg];
@@ -248,6 +250,7 @@ int exec_stmt(xg_kernel* k, int32_t sid, const void* params) {
   const int32_t uni = uniform_refs(k, d) ? XG_F_UNIFORM : 0;
   H.dom.flags = uni;
   if (mode == XG_MODE_INPLACE) {
+    H.dom.flags |= stream_flag(G);  // 3-D box tiles of large grids store non-temporally
     fill_view(H.out, G, G->levels[0]);
     return do_launch(k, sid, d.fn_main, blocks_for_dom(d.dims, H.dom, tile3_cols(d)), kThreads, args);
   }
