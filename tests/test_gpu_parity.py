@@ -100,10 +100,12 @@ def test_cavity_fp32_headline_tile_on_live_data(single):
         assert_bitexact(*pairs[c], levels=[0])
 
 
-@pytest.mark.parametrize("n", [34, 130])
+@pytest.mark.parametrize("n", [34, 130, 150])
 @pytest.mark.parametrize("prec", ["double", "float"])
 def test_heat_3d(prec, n, request):
-    """130^3: three column tiles (the last partial) and two groups of row bands."""
+    """130^3: several column tiles (the last partial) and, in FP32, two groups
+    of 16 row bands; 150^3: 37 bands, a partial last group in both precisions
+    (FP64's 128-column tiles group 32 bands)."""
     xg = request.getfixturevalue("double" if prec == "double" else "single")
     from xgrid_amd import workloads as W
     steps = 25 if n < 100 else 6

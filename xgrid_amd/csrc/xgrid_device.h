@@ -248,8 +248,10 @@ struct TileCells {
        * 512^3 FP32), inside the 4 MB L2 of the XCD the group runs on, and a
        * band's halo rows by the band before it, n2 tiles earlier.  With row
        * bands slowest, the reuse distance of a halo row was a whole band
-       * (4 MB there): 282 against 312 us with aligned columns. */
-      constexpr int64_t G = 16;
+       * (4 MB there): 282 against 312 us with aligned columns.  128-column
+       * tiles (FP64 staged) group 32 bands: 512^3 FP64 417 against 421-424 us
+       * with 16 and 431 with 8; 256-column FP32 tiles were 1 % slower with 32. */
+      constexpr int64_t G = E2 == 128 ? 32 : 16;
       const int64_t n2 = (hi[2] - a2 + E2 - 1) / E2;
       const int64_t n1 = (hi[1] - lo[1] + TL::E1 - 1) / TL::E1;
       const int64_t n0 = (hi[0] - lo[0] + TL::E0 * TL::CH - 1) / (TL::E0 * TL::CH);
