@@ -147,6 +147,29 @@ def test_heat_3d_full_size(prec, request):
     assert np.array_equal(got.view(np.uint8), want.view(np.uint8))
 
 
+def test_heat_3d_unstaged_above_cache_size(double, monkeypatch):
+    """A grid larger than the 256 MB last-level cache makes 3-D box tiles store
+    non-temporally.  The staged tiles are covered at 512^3 above; here staging
+    is off (XGRID_STAGE3=0), so the plain tile's non-temporal variant runs:
+    330^3 FP64 (287 MB), 2 steps, bit for bit against the numpy restatement."""
+    from xgrid_amd import workloads as W
+    from xgrid_amd.backend import lower_kernel
+    monkeypatch.setenv("XGRID_STAGE3", "0")
+    n, steps, nu = 330, 2, 0.1
+    assert n ** 3 * 8 > 256 << 20
+    u0, mask, dx = W.heat_3d_setup(n, np.float64, seed=5)
+    dt = 0.1 * dx * dx / nu
+    k = W.make_heat_3d()
+    assert lower_kernel(k)[0].stmts[0].tile3_cols == 64          # unstaged
+    g = double.Grid((n, n, n), float)
+    g.now[...] = u0
+    g.boundary[...] = mask
+    for _ in range(steps):
+        k(g, nu, dt, dx)
+    want = R.heat_3d(u0, nu, dt, dx, steps, np.float64)
+    assert np.array_equal(g.now.view(np.uint8), want.view(np.uint8))
+
+
 def test_convection_and_diffusion_2d(double):
     from xgrid_amd import workloads as W
     nx = 41
