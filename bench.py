@@ -326,8 +326,8 @@ def leg_batching(args) -> dict:
         # the faster is the baseline: going from instance to instance, each instance's
         # graphs are captured again after its predecessors evicted them, which costs the
         # replays' gain when the device is the bottleneck.
-        # With graph replay switched off process-wide (XGRID_GRAPHS=0, as for a profiled
-        # run), only the graphs-off pass runs.
+        # With graph replay switched off process-wide (XGRID_GRAPHS=0), only the graphs-off
+        # pass runs.
         plain_times = {}
         for graphs in ((True, False) if xgrid_amd.get_config().graphs else (False,)):
             for op in ms.ops.values():
