@@ -136,19 +136,30 @@ int xg_module_unload(xg_module* m);
 /* Storage of one Grid (xgrid/xgrid/__init__.py:22-41): shape, element size,
  * levels allocated on demand. */
 int xg_grid_create(int32_t dims, const int32_t* shape, int32_t elem_bytes, xg_grid** out);
+/* Releases the grid's memory behind the kernels already queued (no wait for them). */
 int xg_grid_destroy(xg_grid* g);
 /* Grow with zero-filled levels or truncate to `depth` (Grid._extend_time, :43-47). */
 int xg_grid_set_depth(xg_grid* g, int32_t depth);
 int32_t xg_grid_depth(xg_grid* g);
 /* Rotate: the oldest level becomes level 0 (Grid._op_invoke, :49-54). */
 int xg_grid_tick(xg_grid* g);
-/* Host <-> device copies of one level (Grid.now / fill, :70-80).  Blocking. */
+/* Host <-> device copies of one level (Grid.now / fill, :70-80).  Blocking.  A
+ * grid that no kernel call has used yet is uploaded without waiting for the
+ * queued kernels; otherwise the upload waits for them (they may read it). */
 int xg_grid_upload(xg_grid* g, int32_t level, const void* host);
 int xg_grid_download(xg_grid* g, int32_t level, void* host);
+/* Copy one level to the host once the work before `event` (xg_event_record) is
+ * done, without waiting for kernels queued after it, which keep running.  No
+ * reference counterpart: the serving loop reads a finished request's fields
+ * while the next tick runs.  The caller ensures that nothing queued after the
+ * event writes this grid.  Blocking. */
+int xg_grid_download_after(xg_grid* g, int32_t level, void* host, void* event);
 /* Install the int32 boundary mask (Grid.boundary, :41) and plan each class. */
 int xg_grid_set_mask(xg_grid* g, const int32_t* mask);
 /* Plan of one class: kind (XG_DOM_*), cell count, box. */
 int xg_grid_class_info(xg_grid* g, int32_t klass, int32_t* kind, int64_t* count, int32_t* lo, int32_t* hi);
+/* Device address of one level.  Work the caller queues with it on its own streams
+ * is waited for before the grid's memory is released or replaced. */
 void* xg_grid_level_ptr(xg_grid* g, int32_t level);
 
 /* ---- kernels: one compiled @xgrid.kernel ------------------------------- */

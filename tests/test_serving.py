@@ -36,6 +36,11 @@ class FakeSolver:
 
 
 class FakeTimer:
+    def __init__(self, log=None):
+        self.log = log if log is not None else []
+        self.marks = 0
+        self.live = set()
+
     def begin(self):
         pass
 
@@ -44,6 +49,18 @@ class FakeTimer:
 
     def sync(self):
         pass
+
+    def mark(self):
+        self.marks += 1
+        self.live.add(self.marks)
+        self.log.append(("mark", self.marks))
+        return self.marks
+
+    def wait(self, event):
+        assert event in self.live
+
+    def release(self, event):
+        self.live.remove(event)
 
     def busy_seconds(self):
         return 0.0
@@ -138,3 +155,55 @@ def test_serving_trace_matches_configured_distribution():
     assert arr == sorted(arr) and 1.5 < arr[-1] < 2.5          # about 400 / 200 s
     assert all(128 <= s.n <= 2048 and 100 <= s.steps <= 2000 and 0.001 <= s.nu <= 0.1 for s in t)
     assert all(s.kind == "cavity" for s in t)
+
+
+def test_finished_requests_are_read_while_the_next_tick_runs():
+    """A request leaves the batch in the tick its steps end; its fields are read in
+    the next iteration, after the next tick is queued, through the event marked
+    right after its last tick (so the copy waits for that tick only).  Every
+    request is read once and every event is released."""
+    import numpy as np
+    log = []
+
+    class Grid:
+        def __init__(self, rid):
+            self.rid = rid
+
+        def _now_after(self, event):
+            log.append(("read", self.rid, event))
+            return np.zeros(2)
+
+    class Solver(FakeSolver):
+        def add(self, spec):
+            log.append(("add", spec.rid))
+            return super().add(spec)
+
+        def step(self, k):
+            super().step(k)
+            log.append(("step", k))
+
+        def remove(self, h):
+            spec = self.members.pop(h)
+            log.append(("remove", spec.rid))
+            return [Grid(spec.rid) for _ in range(4 if spec.kind == "cavity" else 1)]
+
+    trace = _trace(9)
+    fake = Solver()
+    timer = FakeTimer(log)
+    results, _ = Server("float", hbm_budget=1 << 30, steps_per_tick=3, solver=fake, timer=timer).run(trace)
+    assert sorted(results) == [s.rid for s in trace] and not timer.live
+    for s in trace:
+        assert fake.steps[s.rid] == s.steps
+        assert set(results[s.rid].fields) == set("bpuv")
+        i_rm = log.index(("remove", s.rid))
+        reads = [i for i, e in enumerate(log) if e[0] == "read" and e[1] == s.rid]
+        assert len(reads) == 4
+        ev = log[reads[0]][2]
+        i_mark = log.index(("mark", ev))
+        # marked after its last tick, before it left the batch, and read one iteration later
+        assert i_mark < i_rm < reads[0]
+        assert any(e[0] == "mark" and e[1] > ev for e in log[i_rm:reads[0]])
+        # a request in the batch during the reading iteration got a tick before the read
+        later_steps = [e for e in log[i_rm:reads[0]] if e[0] == "step"]
+        others = any(log.index(("add", t.rid)) < reads[0] < log.index(("remove", t.rid)) for t in trace)
+        assert bool(later_steps) == others

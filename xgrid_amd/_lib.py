@@ -78,6 +78,7 @@ _SIGS = {
     "xg_grid_tick": ([c_void_p], c_int),
     "xg_grid_upload": ([c_void_p, c_int32, c_void_p], c_int),
     "xg_grid_download": ([c_void_p, c_int32, c_void_p], c_int),
+    "xg_grid_download_after": ([c_void_p, c_int32, c_void_p, c_void_p], c_int),
     "xg_grid_set_mask": ([c_void_p, c_void_p], c_int),
     "xg_grid_class_info": ([c_void_p, c_int32, POINTER(c_int32), POINTER(c_int64), POINTER(c_int32),
                             POINTER(c_int32)], c_int),

@@ -82,9 +82,12 @@ struct xg_batch {
   std::vector<PendingLaunch> launches;
   void* d_table = nullptr;
   size_t d_table_bytes = 0;
-  void* h_stage[2] = {nullptr, nullptr};
-  size_t h_stage_bytes[2] = {0, 0};
-  hipEvent_t stage_ev[2] = {nullptr, nullptr};
+  // Pinned staging buffers of the launch tables, used in turn: the host may run
+  // kStages calls ahead of the device before it waits for a buffer's last copy.
+  static constexpr int kStages = 8;
+  void* h_stage[kStages] = {};
+  size_t h_stage_bytes[kStages] = {};
+  hipEvent_t stage_ev[kStages] = {};
   int stage_i = 0;
   std::vector<BatchGraph> cache;
   uint64_t clock = 0;
@@ -166,7 +169,7 @@ int flush_direct(xg_batch* b) {
     b->d_table_bytes = cap;
   }
   const int si = b->stage_i;
-  b->stage_i ^= 1;
+  b->stage_i = (si + 1) % xg_batch::kStages;
   if (!b->stage_ev[si]) XG_HIP(hipEventCreateWithFlags(&b->stage_ev[si], hipEventDisableTiming));
   else XG_HIP(hipEventSynchronize(b->stage_ev[si]));   // the last copy out of this buffer is done
   if (bytes > b->h_stage_bytes[si]) {
@@ -444,7 +447,7 @@ int run_loop(xg_batch* b, int32_t lid, int64_t trip, const std::vector<int>& idx
       xg_kernel* k = b->inst[(size_t)idx[(size_t)t]].k;
       if (!k->d_fail) {
         void* p = nullptr;
-        XG_HIP(hipMalloc(&p, sizeof(int32_t)));
+        if (int rc = dev_alloc(sizeof(int32_t), true, &p)) return rc;
         k->d_fail = (int32_t*)p;
       }
       XG_HIP(hipMemsetAsync(k->d_fail, 0, sizeof(int32_t), g_dev.stream));
@@ -734,7 +737,7 @@ int xg_batch_destroy(xg_batch* b) {
   if (g_dev.ready) (void)hipStreamSynchronize(g_dev.stream);
   for (auto& g : b->cache) destroy_graph(g);
   if (b->d_table) (void)hipFree(b->d_table);
-  for (int i = 0; i < 2; ++i) {
+  for (int i = 0; i < xg_batch::kStages; ++i) {
     if (b->h_stage[i]) (void)hipHostFree(b->h_stage[i]);
     if (b->stage_ev[i]) (void)hipEventDestroy(b->stage_ev[i]);
   }

@@ -31,7 +31,9 @@ inline int fail(int code, const std::string& msg) {
 
 struct Device {
   int id = -1;
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr;   // every kernel launch
+  hipStream_t copy = nullptr;     // grid memory set-up and copies that need not wait for the kernels
+  hipMemPool_t pool = nullptr;    // grid and plan memory (dev_alloc / dev_free)
   bool ready = false;
   int cus = 0;
   int64_t hbm = 0;
@@ -47,8 +49,13 @@ inline int check_ready() {
 
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
-// hipMalloc + zero fill on the backend stream
-int alloc_zero(size_t bytes, void** out);
+// Grid and plan memory: taken from the device's pool on the copy stream (cleared
+// there when `zero`) and ready for any work enqueued after the call returns;
+// released on the kernel stream behind the work already queued there.  Neither
+// waits for the kernel stream.
+int dev_alloc(size_t bytes, bool zero, void** out);
+void dev_free(void* p);
+inline int alloc_zero(size_t bytes, void** out) { return dev_alloc(bytes, true, out); }
 
 // Look up this device's address of the reductions' static scratch (device
 // `device` current); called once per device by xg_init.
@@ -76,6 +83,8 @@ struct xg_grid {
   int32_t* d_mask = nullptr;
   uint64_t mask_version = 0;
   std::unordered_map<int32_t, ClassPlan> plans;
+  bool bound = false;     // a kernel plan has used it (or its level pointer was handed out)
+  bool exported = false;  // its level pointer was handed out: callers may use it on their own streams
 
   size_t bytes() const { return (size_t)n * (size_t)elem; }
   const ClassPlan& plan(int32_t k) const {

@@ -20,6 +20,9 @@ constexpr int64_t kGraphMinLaunches = 4;    // a call with fewer launches runs t
 // stores: a later launch cannot find their cells in cache anyway, and the hint
 // leaves L2 to the cells the launch still reads (halo rows and planes).
 constexpr size_t kStreamBytes = (size_t)256 << 20;
+// Freed grid memory the device pool keeps for reuse (beyond it, memory goes back at
+// the next synchronisation).
+constexpr uint64_t kPoolKeepBytes = (uint64_t)16 << 30;
 
 // kRecord: the stub's statements are recorded for a batched launch instead of run
 enum CallMode { kDirect = 0, kCapture = 1, kReplay = 2, kRecord = 3 };

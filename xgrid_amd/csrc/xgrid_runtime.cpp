@@ -17,6 +17,7 @@
 #include <unordered_map>
 #include <utility>
 #include <climits>
+#include <mutex>
 #include <vector>
 
 #include "xgrid_internal.h"
@@ -27,14 +28,44 @@ namespace xgi {
 thread_local std::string t_err;
 Device g_dev;
 
-int alloc_zero(size_t bytes, void** out) {
+/* A serving loop admits and retires grids while other requests' kernels run:
+ * hipFree waits for the whole device, and so did the stream synchronisations
+ * around allocation and copies, so each admission or retirement left the
+ * device idle until the host queued work again.  Grid and
+ * plan memory therefore comes from a stream-ordered pool.  It is taken (and
+ * cleared) on the copy stream, which the host then waits for, so it is ready
+ * for any work enqueued afterwards on either stream; it is given back on the
+ * kernel stream, behind the kernels already queued there that may use it.  The
+ * pool never makes the copy stream wait for the kernel stream (internal
+ * dependencies off): a block freed behind queued kernels is reused only once
+ * those kernels are done. */
+int dev_alloc(size_t bytes, bool zero, void** out) {
   void* p = nullptr;
-  hipError_t e = hipMalloc(&p, bytes ? bytes : 1);
-  if (e != hipSuccess) return fail(XG_ERR_MEM, std::string("hipMalloc of ") + std::to_string(bytes) + " bytes failed: " + hipGetErrorString(e));
-  e = hipMemsetAsync(p, 0, bytes ? bytes : 1, g_dev.stream);
-  if (e != hipSuccess) return fail(XG_ERR_HIP, std::string("hipMemsetAsync failed: ") + hipGetErrorString(e));
+  const size_t n = bytes ? bytes : 1;
+  hipError_t e = hipMallocFromPoolAsync(&p, n, g_dev.pool, g_dev.copy);
+  if (e != hipSuccess) return fail(XG_ERR_MEM, std::string("device allocation of ") + std::to_string(bytes) + " bytes failed: " + hipGetErrorString(e));
+  if (zero) e = hipMemsetAsync(p, 0, n, g_dev.copy);
+  if (e == hipSuccess) e = hipStreamSynchronize(g_dev.copy);
+  if (e != hipSuccess) {
+    (void)hipFreeAsync(p, g_dev.copy);
+    return fail(XG_ERR_HIP, std::string("device allocation set-up failed: ") + hipGetErrorString(e));
+  }
   *out = p;
   return XG_OK;
+}
+
+/* While the kernel stream records a graph (another thread inside a call), a free
+ * enqueued there would become part of the graph: such frees wait in a list until
+ * the next release made outside a capture. */
+void dev_free(void* p) {
+  static std::mutex mu;
+  static std::vector<void*> held;
+  std::lock_guard<std::mutex> lock(mu);
+  if (p) held.push_back(p);
+  hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(g_dev.stream, &st) != hipSuccess || st != hipStreamCaptureStatusNone) return;
+  for (void* q : held) (void)hipFreeAsync(q, g_dev.stream);
+  held.clear();
 }
 
 }  // namespace xgi
@@ -47,7 +78,7 @@ uint64_t g_mask_version = 1;
 
 void free_plans(xg_grid* g) {
   for (auto& kv : g->plans)
-    if (kv.second.d_list) (void)hipFree(kv.second.d_list);
+    dev_free(kv.second.d_list);
   g->plans.clear();
 }
 
@@ -494,8 +525,10 @@ int api_begin(xg_kernel* k, xg_grid* const* grids, int32_t ngrids, const void* k
   if (int rc = check_ready()) return set_status(k, rc);
   if (ngrids != k->ngrids) return set_status(k, fail(XG_ERR_ARG, "grid argument count mismatch"));
   k->grids.assign(grids, grids + ngrids);
-  for (auto* G : k->grids)
+  for (auto* G : k->grids) {
     if (!G) return set_status(k, fail(XG_ERR_ARG, "null grid handle"));
+    G->bound = true;
+  }
   if (int rc = ensure_scratch(k)) return set_status(k, rc);
   if (k->record_next) {
     k->mode = kRecord;
@@ -752,8 +785,8 @@ int api_loop(xg_kernel* k, int32_t lid, int64_t trip, const void* const* params,
   if (check) {
     if (!k->d_fail) {
       void* p = nullptr;
-      if (hipMalloc(&p, sizeof(int32_t)) != hipSuccess) {
-        set_status(k, fail(XG_ERR_MEM, "hipMalloc failed"));
+      if (int e = dev_alloc(sizeof(int32_t), true, &p)) {
+        set_status(k, e);
         return 1;
       }
       k->d_fail = (int32_t*)p;
@@ -828,6 +861,18 @@ int xg_init(int device) {
   hipDeviceProp_t prop;
   XG_HIP(hipGetDeviceProperties(&prop, device));
   XG_HIP(hipStreamCreateWithFlags(&g_dev.stream, hipStreamNonBlocking));
+  XG_HIP(hipStreamCreateWithFlags(&g_dev.copy, hipStreamNonBlocking));
+  hipMemPoolProps props;
+  std::memset(&props, 0, sizeof props);
+  props.allocType = hipMemAllocationTypePinned;
+  props.handleTypes = hipMemHandleTypeNone;
+  props.location.type = hipMemLocationTypeDevice;
+  props.location.id = device;
+  XG_HIP(hipMemPoolCreate(&g_dev.pool, &props));
+  uint64_t keep = kPoolKeepBytes;   // freed memory the pool keeps for reuse instead of returning it
+  XG_HIP(hipMemPoolSetAttribute(g_dev.pool, hipMemPoolAttrReleaseThreshold, &keep));
+  int no_waits = 0;
+  XG_HIP(hipMemPoolSetAttribute(g_dev.pool, hipMemPoolReuseAllowInternalDependencies, &no_waits));
   g_dev.id = device;
   g_dev.cus = prop.multiProcessorCount;
   g_dev.hbm = (int64_t)prop.totalGlobalMem;
@@ -948,10 +993,12 @@ int xg_grid_create(int32_t dims, const int32_t* shape, int32_t elem_bytes, xg_gr
 
 int xg_grid_destroy(xg_grid* g) {
   if (!g) return XG_OK;
-  if (g_dev.ready) (void)hipStreamSynchronize(g_dev.stream);
-  for (void* l : g->levels) (void)hipFree(l);
-  if (g->scratch) (void)hipFree(g->scratch);
-  if (g->d_mask) (void)hipFree(g->d_mask);
+  // the memory is released behind the kernels queued on the kernel stream; work that
+  // a caller queued elsewhere on a handed-out level pointer is waited for first
+  if (g->exported && g_dev.ready) (void)hipDeviceSynchronize();
+  for (void* l : g->levels) dev_free(l);
+  dev_free(g->scratch);
+  dev_free(g->d_mask);
   free_plans(g);
   delete g;
   return XG_OK;
@@ -967,9 +1014,9 @@ int xg_grid_set_depth(xg_grid* g, int32_t depth) {
     g->levels.push_back(p);
   }
   if ((int32_t)g->levels.size() > depth) {
-    XG_HIP(hipStreamSynchronize(g_dev.stream));
+    if (g->exported) XG_HIP(hipDeviceSynchronize());
     while ((int32_t)g->levels.size() > depth) {
-      (void)hipFree(g->levels.back());
+      dev_free(g->levels.back());
       g->levels.pop_back();
     }
   }
@@ -987,8 +1034,11 @@ int xg_grid_tick(xg_grid* g) {
 int xg_grid_upload(xg_grid* g, int32_t level, const void* host) {
   if (int rc = check_ready()) return rc;
   if (!g || !host || level < 0 || level >= (int32_t)g->levels.size()) return fail(XG_ERR_ARG, "invalid level");
-  XG_HIP(hipMemcpyAsync(g->levels[(size_t)level], host, g->bytes(), hipMemcpyHostToDevice, g_dev.stream));
-  XG_HIP(hipStreamSynchronize(g_dev.stream));
+  // A grid no kernel has used yet is written beside the queued kernels (a request
+  // being admitted while others run); otherwise behind them, since they may read it.
+  hipStream_t s = g->bound ? g_dev.stream : g_dev.copy;
+  XG_HIP(hipMemcpyAsync(g->levels[(size_t)level], host, g->bytes(), hipMemcpyHostToDevice, s));
+  XG_HIP(hipStreamSynchronize(s));
   return XG_OK;
 }
 
@@ -1000,8 +1050,20 @@ int xg_grid_download(xg_grid* g, int32_t level, void* host) {
   return XG_OK;
 }
 
+int xg_grid_download_after(xg_grid* g, int32_t level, void* host, void* event) {
+  if (int rc = check_ready()) return rc;
+  if (!g || !host || !event || level < 0 || level >= (int32_t)g->levels.size())
+    return fail(XG_ERR_ARG, "invalid level or event");
+  // on the copy stream: the kernels queued after the event keep the device busy
+  XG_HIP(hipStreamWaitEvent(g_dev.copy, (hipEvent_t)event, 0));
+  XG_HIP(hipMemcpyAsync(host, g->levels[(size_t)level], g->bytes(), hipMemcpyDeviceToHost, g_dev.copy));
+  XG_HIP(hipStreamSynchronize(g_dev.copy));
+  return XG_OK;
+}
+
 void* xg_grid_level_ptr(xg_grid* g, int32_t level) {
   if (!g || level < 0 || level >= (int32_t)g->levels.size()) return nullptr;
+  g->bound = g->exported = true;
   return g->levels[(size_t)level];
 }
 
@@ -1035,7 +1097,9 @@ int xg_grid_set_mask(xg_grid* g, const int32_t* mask) {
           if (idx[d] > s.hi[d]) s.hi[d] = idx[d];
         }
       }
-  XG_HIP(hipStreamSynchronize(g_dev.stream));
+  // New lists and mask go to new memory (written on the copy stream); the old ones are
+  // released behind the queued kernels that may still read them.
+  if (g->exported) XG_HIP(hipDeviceSynchronize());
   free_plans(g);
   std::unordered_map<int32_t, std::vector<int32_t>> lists;
   for (auto& kv : stats) {
@@ -1065,17 +1129,20 @@ int xg_grid_set_mask(xg_grid* g, const int32_t* mask) {
     for (auto& kv : lists) {
       ClassPlan& p = g->plans[kv.first];
       void* d = nullptr;
-      XG_HIP(hipMalloc(&d, kv.second.size() * sizeof(int32_t)));
-      XG_HIP(hipMemcpy(d, kv.second.data(), kv.second.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+      if (int rc = dev_alloc(kv.second.size() * sizeof(int32_t), false, &d)) return rc;
       p.d_list = (int32_t*)d;
+      XG_HIP(hipMemcpyAsync(d, kv.second.data(), kv.second.size() * sizeof(int32_t), hipMemcpyHostToDevice, g_dev.copy));
     }
   }
-  if (!g->d_mask) {
+  dev_free(g->d_mask);
+  g->d_mask = nullptr;
+  {
     void* d = nullptr;
-    XG_HIP(hipMalloc(&d, (size_t)g->n * sizeof(int32_t)));
+    if (int rc = dev_alloc((size_t)g->n * sizeof(int32_t), false, &d)) return rc;
     g->d_mask = (int32_t*)d;
   }
-  XG_HIP(hipMemcpy(g->d_mask, mask, (size_t)g->n * sizeof(int32_t), hipMemcpyHostToDevice));
+  XG_HIP(hipMemcpyAsync(g->d_mask, mask, (size_t)g->n * sizeof(int32_t), hipMemcpyHostToDevice, g_dev.copy));
+  XG_HIP(hipStreamSynchronize(g_dev.copy));   // the host lists and the caller's mask are read by then
   g->h_mask.assign(mask, mask + g->n);
   g->mask_version = g_mask_version++;
   return XG_OK;
@@ -1155,7 +1222,7 @@ int xg_kernel_destroy(xg_kernel* k) {
     (void)hipEventDestroy(ev.a);
     (void)hipEventDestroy(ev.b);
   }
-  if (k->d_fail) (void)hipFree(k->d_fail);
+  dev_free(k->d_fail);
   delete k;
   return XG_OK;
 }
@@ -1337,7 +1404,7 @@ int xg_copy_probe(int64_t bytes, int32_t iters, double* gbps, double* ms_per_ite
   if (rc) return rc;
   rc = alloc_zero((size_t)n * 32, &b);
   if (rc) {
-    (void)hipFree(a);
+    dev_free(a);
     return rc;
   }
   const int blocks = (int)std::min<int64_t>(ceil_div(n, 256), 256 * 64);
@@ -1354,8 +1421,8 @@ int xg_copy_probe(int64_t bytes, int32_t iters, double* gbps, double* ms_per_ite
   XG_HIP(hipEventElapsedTime(&ms, e0, e1));
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  (void)hipFree(a);
-  (void)hipFree(b);
+  dev_free(a);
+  dev_free(b);
   const double per = ms / iters;
   if (ms_per_iter) *ms_per_iter = per;
   if (gbps) *gbps = 2.0 * (double)n * 32.0 / (per * 1e-3) / 1e9;

@@ -156,6 +156,21 @@ class Grid:
             _lib.check(_lib.load().xg_grid_download(self._handle, t, lvl.host.ctypes.data), "xg_grid_download")
             lvl.host_valid = True
 
+    def _now_after(self, event) -> np.ndarray:
+        """`now`, read once the device work recorded before `event` (``xg_event_record``)
+        is done; kernels queued after the event keep running during the copy and must
+        not write this grid.  The serving loop reads a finished request this way while
+        the next tick runs."""
+        lvl = self._levels[0]
+        if not lvl.host_valid:
+            if lvl.host is None:
+                lvl.host = np.empty(self.shape, dtype=self.numpy_dtype)
+            _lib.check(_lib.load().xg_grid_download_after(self._handle, 0, lvl.host.ctypes.data, event),
+                       "xg_grid_download_after")
+            lvl.host_valid = True
+        lvl.dev_valid = False
+        return lvl.host
+
     def _sync_for_launch(self):
         """Make every level and the mask current on the device; returns the handle."""
         h = self._device()

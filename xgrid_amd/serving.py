@@ -60,6 +60,19 @@ class _DeviceTimer:
     def sync(self):
         _lib.check(self.lib.xg_synchronize(), "xg_synchronize")
 
+    def mark(self):
+        """An event after the work queued so far (release it with `release`)."""
+        return self._record()
+
+    def wait(self, event):
+        """Wait until the work before `event` is done."""
+        import ctypes
+        ms = ctypes.c_double()
+        _lib.check(self.lib.xg_event_elapsed(event, event, ctypes.byref(ms)), "xg_event_elapsed")
+
+    def release(self, event):
+        self.lib.xg_event_destroy(event)
+
     def busy_seconds(self) -> float:
         self._drain(len(self.pending))
         t, self.total_ms = self.total_ms, 0.0
@@ -131,12 +144,19 @@ class Server:
     def run(self, trace: list, time_scale: float = 1.0, on_done=None) -> tuple:
         """Serve `trace` (arrival times in seconds, scaled by `time_scale`) in real time;
         returns (results by request id, summary).  `on_done(result, n_done)` is called
-        as each request completes."""
+        as each request completes.
+
+        A request whose steps are done leaves the batch at once, and its fields are
+        read in the next iteration, after the next tick is queued: the copy waits
+        only for the tick it finished in, and the device runs the next one
+        meanwhile.  Its memory is released behind the queued kernels, so neither
+        admitting nor retiring a request waits for the device."""
         pending = deque(sorted(trace, key=lambda s: s.arrival))
         queue: deque = deque()
         active = {}                      # handle -> [spec, steps done, admitted time]
+        retiring = []                    # [spec, admitted time, grids, event after its last tick]
         used = 0
-        resident = 0                     # cells of the admitted requests
+        resident = 0                     # cells of the requests in the batch
         results = {}
         for s in trace:
             if footprint(s, self.elem) > self.budget:
@@ -144,7 +164,7 @@ class Server:
         self.peak_used = self.peak_active = self.ticks = 0
         t0 = time.perf_counter()
         now = lambda: time.perf_counter() - t0        # noqa: E731
-        while pending or queue or active:
+        while pending or queue or active or retiring:
             t = now()
             while pending and pending[0].arrival * time_scale <= t:
                 queue.append(pending.popleft())
@@ -157,17 +177,33 @@ class Server:
                 active[h] = [s, 0, now()]
                 self.peak_used = max(self.peak_used, used)
                 self.peak_active = max(self.peak_active, len(active))
-            if not active:
+            if not active and not retiring:
                 if pending:
                     time.sleep(max(0.0, pending[0].arrival * time_scale - now()))
                 continue
-            k = max(0, min(self.steps_per_tick, min(s.steps - d for s, d, _ in active.values())))
+            k = max(0, min(self.steps_per_tick, min((s.steps - d for s, d, _ in active.values()), default=0)))
             if k:
                 # no wait here: the host prepares the next tick while the device runs this one
                 self.timer.begin()
                 self.ms.step(k)
                 self.timer.end()
                 self.ticks += 1
+            event = self.timer.mark()
+            for s, adm, grids, after in retiring:
+                fields = {}
+                if self.keep_fields:           # the copy waits for the request's last tick only
+                    names = "bpuv" if s.kind == "cavity" else "u"
+                    fields = {c: g._now_after(after).copy() for c, g in zip(names, grids)}
+                else:
+                    self.timer.wait(after)
+                del grids
+                used -= footprint(s, self.elem)
+                results[s.rid] = Result(s.rid, s.arrival * time_scale, adm, now(), fields)
+                if on_done is not None:
+                    on_done(results[s.rid], len(results))
+            for *_, after in retiring:
+                self.timer.release(after)
+            retiring = []
             finished = []
             for h, rec in active.items():
                 rec[1] += k
@@ -175,19 +211,10 @@ class Server:
                     finished.append(h)
             for h in finished:
                 s, _, adm = active.pop(h)
-                grids = self.ms.remove(h)
-                fields = {}
-                if self.keep_fields:           # the copy waits for the request's last step
-                    names = "bpuv" if s.kind == "cavity" else "u"
-                    fields = {c: g.now.copy() for c, g in zip(names, grids)}
-                else:
-                    self.timer.sync()
-                del grids
-                used -= footprint(s, self.elem)
+                retiring.append([s, adm, self.ms.remove(h), event])
                 resident -= cells(s)
-                results[s.rid] = Result(s.rid, s.arrival * time_scale, adm, now(), fields)
-                if on_done is not None:
-                    on_done(results[s.rid], len(results))
+            if not finished:
+                self.timer.release(event)
         self.timer.sync()
         wall = now()
         busy = self.timer.busy_seconds()
