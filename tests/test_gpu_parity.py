@@ -204,3 +204,46 @@ def test_convection_and_diffusion_2d(double):
     d, h = grid_pair(double, (70, 70), float, W.box_walls((70, 70)), u0)
     run_pair(k, (d, 0.05, 1e-4, 0.02), (h, 0.05, 1e-4, 0.02), steps=30)
     assert_bitexact(d, h)
+
+
+def test_grids_dropped_behind_queued_kernels(double):
+    """Grid memory is released behind the kernels already queued and taken on a
+    copy stream without waiting for them.  Grids dropped while their calls may
+    still be queued, and new grids of the same size made and uploaded at once
+    (likely reusing the released blocks and addresses), each give the numpy
+    restatement's result bit for bit, and so does a grid kept across the churn
+    (its calls are replayed graphs, keyed by grid identity, not address)."""
+    import gc
+    from xgrid_amd import workloads as W
+    n, nu = 768, 0.1
+    dx = 2.0 / (n - 1)
+    dt = 0.2 * dx * dx / nu
+    k = W.make_diffusion_2d()
+    mask = np.ones((n, n), np.int32)
+    mask[1:-1, 1:-1] = 0
+    rng = np.random.default_rng(7)
+
+    def fresh(u0):
+        g = double.Grid((n, n), float)
+        g.now[...] = u0
+        g.boundary[...] = mask
+        return g
+
+    u_keep = rng.random((n, n))
+    keep = fresh(u_keep)
+    kept = 0
+    for _ in range(4):
+        a = fresh(rng.random((n, n)))
+        for _ in range(30):
+            k(a, nu, dt, dx)
+        k(keep, nu, dt, dx)
+        kept += 1
+        del a                        # its calls may still be queued
+        gc.collect()
+        u1 = rng.random((n, n))
+        b = fresh(u1)
+        for _ in range(7):
+            k(b, nu, dt, dx)
+        assert np.array_equal(b.now, R.diffusion_2d(u1, nu, dt, dx, 7))
+        del b
+    assert np.array_equal(keep.now, R.diffusion_2d(u_keep, nu, dt, dx, kept))
