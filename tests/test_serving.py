@@ -187,7 +187,10 @@ def test_finished_requests_are_read_while_the_next_tick_runs():
             log.append(("remove", spec.rid))
             return [Grid(spec.rid) for _ in range(4 if spec.kind == "cavity" else 1)]
 
-    trace = _trace(9)
+    # the last three arrive together with equal steps: they finish in one tick and
+    # share its event, which is released once
+    trace = _trace(9) + [W.InstanceSpec("cavity", 64, 0.05, 1e-5, steps=5, arrival=0.01, rid=9 + i)
+                         for i in range(3)]
     fake = Solver()
     timer = FakeTimer(log)
     results, _ = Server("float", hbm_budget=1 << 30, steps_per_tick=3, solver=fake, timer=timer).run(trace)

@@ -1102,6 +1102,11 @@ int xg_grid_set_mask(xg_grid* g, const int32_t* mask) {
   if (g->exported) XG_HIP(hipDeviceSynchronize());
   free_plans(g);
   std::unordered_map<int32_t, std::vector<int32_t>> lists;
+  // leaving early (an allocation or copy failed), copies already queued from `lists`
+  // finish before it is freed (the guard goes out of scope first)
+  struct CopiesDone {
+    ~CopiesDone() { (void)hipStreamSynchronize(g_dev.copy); }
+  } copies_done;
   for (auto& kv : stats) {
     ClassPlan p;
     p.count = kv.second.count;

@@ -201,7 +201,7 @@ class Server:
                 results[s.rid] = Result(s.rid, s.arrival * time_scale, adm, now(), fields)
                 if on_done is not None:
                     on_done(results[s.rid], len(results))
-            for *_, after in retiring:
+            for after in {r[3] for r in retiring}:      # requests of one tick share its event
                 self.timer.release(after)
             retiring = []
             finished = []
