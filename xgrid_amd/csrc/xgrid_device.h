@@ -145,10 +145,9 @@ __device__ __forceinline__ T ld(const xg_gview& g, const int32_t* sh, int64_t xc
   return gp<const T>(g.ptr)[k];
 }
 
-/* Write v at dst; NT: non-temporally, so the line is evicted first and a grid
- * larger than the cache does not push out the cells the launch still reads.
- * The choice is a template argument: the compiler merges a run-time branch
- * between the two forms into one plain store. */
+/* Write v at dst; NT: with the non-temporal hint (the line is not kept for
+ * reuse).  The choice is a template argument: the compiler merges a run-time
+ * branch between the two forms into one plain store. */
 template <class V, bool NT>
 __device__ __forceinline__ void put(gptr<V> dst, const V& v) {
   if constexpr (NT) __builtin_nontemporal_store(v, dst);
