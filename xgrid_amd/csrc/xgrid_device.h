@@ -50,8 +50,8 @@ template <> struct Tile<1> { static constexpr int E0 = 1024, E1 = 1, E2 = 1, NC 
 template <> struct Tile<2> { static constexpr int E0 = 4, E1 = 64, E2 = 1, NC = 1, CH = 1; };
 template <> struct Tile<3> { static constexpr int E0 = 4, E1 = 4, E2 = 64, NC = 4, CH = XG_TILE3_CHUNKS; };
 
-/* Every pointer in a launch header, view or batch table is device memory from
- * hipMalloc (global address space).  A pointer read out of memory (a batch
+/* Every pointer in a launch header, view or batch table is device memory
+ * (global address space).  A pointer read out of memory (a batch
  * table entry) is generic to the compiler, which then emits flat accesses:
  * those wait on the LDS counter too and serialize against LDS traffic.  This
  * cast states the address space, so accesses are global loads and stores. */
