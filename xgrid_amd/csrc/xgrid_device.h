@@ -377,27 +377,36 @@ __device__ __forceinline__ void box_cells(const xg_launch_hdr& H, const typename
         constexpr int K = S::CPL;
         const int col = tc.col0 + (int)(threadIdx.x & 63) * K;
         const bool row_fits = col + K <= H.out.shape[2];
+        /* The partly stored lanes' current values are all read before the first
+         * store: read one store at a time, the compiler cannot move a read above
+         * the store before it (same array), and the wave waited for memory once
+         * per pair (the 512^3 FP64 heat update spent 47 of its 419 us there). */
+        vecn<T, K> old[NC / K];
+        if (row_fits) {
+#pragma unroll
+          for (int n = 0; n < NC; n += K) {
+            bool all = true, any = false;
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+              all = all && tc.in[n + k];
+              any = any || tc.in[n + k];
+            }
+            if (any && !all)
+              old[n / K] = *(gptr<const vecn<T, K>>)(gp<T>(H.out.ptr) + lin<3>(H.out.shape, tc.c[n][0], tc.c[n][1], col));
+          }
+        }
 #pragma unroll
         for (int n = 0; n < NC; n += K) {
-          bool all = true, any = false;
+          bool any = false;
 #pragma unroll
-          for (int k = 0; k < K; ++k) {
-            all = all && tc.in[n + k];
-            any = any || tc.in[n + k];
-          }
+          for (int k = 0; k < K; ++k) any = any || tc.in[n + k];
           if (!any) continue;
           if (row_fits) {
             const gptr<vecn<T, K>> dst =
                 (gptr<vecn<T, K>>)(gp<T>(H.out.ptr) + lin<3>(H.out.shape, tc.c[n][0], tc.c[n][1], col));
             vecn<T, K> w;
 #pragma unroll
-            for (int k = 0; k < K; ++k) w[k] = v[n + k];
-            if (!all) {
-              const vecn<T, K> old = *dst;
-#pragma unroll
-              for (int k = 0; k < K; ++k)
-                if (!tc.in[n + k]) w[k] = old[k];
-            }
+            for (int k = 0; k < K; ++k) w[k] = tc.in[n + k] ? v[n + k] : old[n / K][k];
             put<vecn<T, K>, NT>(dst, w);
           } else {
 #pragma unroll
