@@ -370,7 +370,8 @@ __device__ __forceinline__ void box_cells(const xg_launch_hdr& H, const typename
         /* A lane whose CPL cells are only partly stored (the class edge, a hole of a
          * masked class) merges its computed cells into the current values of the
          * others and stores all 16 bytes, so every lane of the wave takes the same
-         * single store.  No other work-item of the launch writes those cells: they
+         * single store (with two columns per lane in a box class it stores its one
+         * cell instead, below).  No other work-item of the launch writes those cells: they
          * are outside the statement's class, or outside the columns another tile
          * covers (tiles start at multiples of their width), and staged stores run
          * only unshifted and in place. */
@@ -380,9 +381,14 @@ __device__ __forceinline__ void box_cells(const xg_launch_hdr& H, const typename
         /* The partly stored lanes' current values are all read before the first
          * store: read one store at a time, the compiler cannot move a read above
          * the store before it (same array), and the wave waited for memory once
-         * per pair (the 512^3 FP64 heat update spent 47 of its 419 us there). */
+         * per pair (the 512^3 FP64 heat update spent 47 of its 419 us there).
+         * With two columns per lane and a box class (no holes), such a lane has
+         * one cell to store and stores just that cell, with no read at all (FP64
+         * heat 390 against 405 us); with four columns its up to three cells
+         * cost more as single stores than the read (FP32 234 against 207 us). */
+        constexpr bool MERGE = MASKED || K > 2;
         vecn<T, K> old[NC / K];
-        if (row_fits) {
+        if (MERGE && row_fits) {
 #pragma unroll
           for (int n = 0; n < NC; n += K) {
             bool all = true, any = false;
@@ -397,16 +403,19 @@ __device__ __forceinline__ void box_cells(const xg_launch_hdr& H, const typename
         }
 #pragma unroll
         for (int n = 0; n < NC; n += K) {
-          bool any = false;
+          bool all = true, any = false;
 #pragma unroll
-          for (int k = 0; k < K; ++k) any = any || tc.in[n + k];
+          for (int k = 0; k < K; ++k) {
+            all = all && tc.in[n + k];
+            any = any || tc.in[n + k];
+          }
           if (!any) continue;
-          if (row_fits) {
+          if (row_fits && (MERGE || all)) {
             const gptr<vecn<T, K>> dst =
                 (gptr<vecn<T, K>>)(gp<T>(H.out.ptr) + lin<3>(H.out.shape, tc.c[n][0], tc.c[n][1], col));
             vecn<T, K> w;
 #pragma unroll
-            for (int k = 0; k < K; ++k) w[k] = tc.in[n + k] ? v[n + k] : old[n / K][k];
+            for (int k = 0; k < K; ++k) w[k] = (!MERGE || tc.in[n + k]) ? v[n + k] : old[n / K][k];
             put<vecn<T, K>, NT>(dst, w);
           } else {
 #pragma unroll
