@@ -319,9 +319,11 @@ class HipKernel:
             if isinstance(t, GridType):
                 if not isinstance(a, Grid):
                     raise TypeError(f"argument '{name}' expects a Grid")
-                if a.dimension != t.dimension or a.element != t.element:
-                    raise TypeError(f"argument '{name}' expects {t!r}, got a grid of {a.element!r} with "
-                                    f"{a.dimension} dimension(s)")
+                if a._typed_for is not t:         # a grid's element and rank never change
+                    if a.dimension != t.dimension or a.element != t.element:
+                        raise TypeError(f"argument '{name}' expects {t!r}, got a grid of {a.element!r} with "
+                                        f"{a.dimension} dimension(s)")
+                    a._typed_for = t
                 cargs.append(a._sync_for_launch())
                 grids.append(a)
             elif isinstance(t, Structure):

@@ -87,6 +87,7 @@ struct xg_kernel {
   std::vector<xgi::Pending> pending;
   std::vector<uint8_t> arena;
   int32_t pending_group = -1;
+  std::vector<xgi::Pending> run_buf;   // flush's working list (kept for its capacity)
   bool graphs = true, fuse = true;
   bool graphs_auto_off = false;
   int64_t graph_misses = 0, graph_hits = 0;

@@ -274,7 +274,7 @@ int exec_stmt(xg_kernel* k, int32_t sid, const void* params) {
   xg_launch_hdr H;
   std::memset(&H, 0, sizeof H);
   fill_dom(H.dom, p, G, d.klass);
-  std::vector<xg_gview> refs;
+  thread_local std::vector<xg_gview> refs;   // kernel arguments are copied at the launch
   int rc = fill_refs(k, d, refs);
   if (rc) return rc;
   void* args[3] = {&H, refs.data(), const_cast<void*>(params)};
@@ -317,13 +317,18 @@ int exec_stmt(xg_kernel* k, int32_t sid, const void* params) {
 int exec_group(xg_kernel* k, int32_t gi, const std::vector<int>& sids, const std::vector<const void*>& params) {
   const xg_group_desc& gd = k->groups[(size_t)gi];
   static const uint8_t zeros[4096] = {0};
-  std::vector<xg_launch_hdr> hdrs((size_t)gd.count);
-  std::vector<std::vector<xg_gview>> refs((size_t)gd.count);
-  std::vector<void*> args;
+  // buffers kept between calls (kernel arguments are copied at the launch)
+  thread_local std::vector<xg_launch_hdr> hdrs;
+  thread_local std::vector<std::vector<xg_gview>> refs;
+  thread_local std::vector<void*> args;
+  thread_local std::vector<const void*> par;
+  hdrs.resize((size_t)gd.count);
+  if (refs.size() < (size_t)gd.count) refs.resize((size_t)gd.count);
   uint32_t en = 0;
+  args.clear();
   args.reserve(1 + 3 * (size_t)gd.count);
   args.push_back(&en);
-  std::vector<const void*> par((size_t)gd.count, zeros);
+  par.assign((size_t)gd.count, zeros);
   for (size_t i = 0; i < sids.size(); ++i) {
     const int pos = sids[i] - gd.first;
     en |= 1u << pos;
@@ -400,7 +405,10 @@ int exec_pair(xg_kernel* k, int32_t a, const void* pa, const void* pb) {
 
 int flush(xg_kernel* k) {
   if (k->pending.empty()) return XG_OK;
-  std::vector<Pending> run;
+  // the run takes the pending list's buffer and gives back its own (empty, with capacity):
+  // a call allocates nothing here once the buffers have grown
+  std::vector<Pending>& run = k->run_buf;
+  run.clear();
   run.swap(k->pending);
   const int32_t gi = k->pending_group;
   k->pending_group = -1;
@@ -413,7 +421,8 @@ int flush(xg_kernel* k) {
     return rc;
   }
   // eligibility: in place, small, no hazard
-  std::vector<char> ok(run.size(), 0);
+  thread_local std::vector<char> ok;
+  ok.assign(run.size(), 0);
   for (size_t i = 0; i < run.size(); ++i) {
     StmtPlan& sp = k->stmts[(size_t)run[i].sid];
     xg_grid* G = k->grids[(size_t)sp.d.store_arg];
@@ -435,8 +444,10 @@ int flush(xg_kernel* k) {
     size_t j = i;
     while (j < run.size() && ok[j]) ++j;
     if (j - i >= 2) {
-      std::vector<int> sids;
-      std::vector<const void*> ps;
+      thread_local std::vector<int> sids;
+      thread_local std::vector<const void*> ps;
+      sids.clear();
+      ps.clear();
       for (size_t t = i; t < j; ++t) {
         sids.push_back(run[t].sid);
         ps.push_back(params_of(run[t]));

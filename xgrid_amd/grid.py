@@ -64,6 +64,7 @@ class Grid:
         self._mask_dirty = False        # the device default is the all-zero mask
         self._handle = None
         self._finalizer = None
+        self._typed_for = None          # the last kernel argument type this grid was checked against
 
     # ------------------------------------------------------------ properties
     @property
@@ -110,7 +111,8 @@ class Grid:
             _lib.check(_lib.load().xg_grid_set_depth(self._handle, len(self._levels)), "xg_grid_set_depth")
 
     def _op_invoke(self, depth: int, tick: bool) -> None:
-        self._extend_time(depth)
+        if len(self._levels) != depth:
+            self._extend_time(depth)
         if tick and len(self._levels) > 1:
             self._levels.insert(0, self._levels.pop())
             if self._handle is not None:

@@ -20,6 +20,7 @@ from .types import BaseType
 # (this thread-local one), never into Operator._ir: what the first call parses is
 # decided then, under the configuration and globals in force at that time.
 _private = threading.local()
+_Grid = None                     # xgrid_amd.grid.Grid, imported on the first kernel call
 _ir_lock = threading.RLock()     # a parse parses the operators it calls
 
 CustomTypecheck = Callable[[list], BaseType]
@@ -51,15 +52,18 @@ class Operator:
 
     def __call__(self, *args: Any) -> Any:
         if self.mode == "kernel":
+            global _Grid
             if self.native is None:
                 from .backend import build_kernel
                 if self._precompiled is not None:
                     self._precompiled.wait()      # its code objects are then in the cache
                 self.native, self.depth = build_kernel(self)
-            from .grid import Grid
+            if _Grid is None:
+                from .grid import Grid as _Grid
+            depth, tick = self.depth, self.tick
             for arg in args:
-                if isinstance(arg, Grid):
-                    arg._op_invoke(self.depth, self.tick)
+                if isinstance(arg, _Grid):
+                    arg._op_invoke(depth, tick)
             return self.native(*args)
         if self.mode == "function":
             return self.func(*args)
