@@ -402,6 +402,38 @@ def test_nontemporal_stores_only_in_3d_box_kernels(double):
     assert flat and not any(nt for _, nt in flat), flat
 
 
+def _lds_loads_unwaited(lines):
+    """Indexes of barriers reached (in code order) after a load into LDS with
+    no wait for the vector-memory counter to reach 0 in between."""
+    bad, pending = [], False
+    for i, ln in enumerate(lines):
+        if ln.startswith("global_load_lds_"):
+            pending = True
+        elif ln.startswith("s_waitcnt") and "vmcnt(0)" in ln:
+            pending = False
+        elif ln.startswith("s_barrier") and pending:
+            bad.append(i)
+    return bad
+
+
+def test_staged_halo_rows_load_into_lds_before_the_barrier(double):
+    """The staged 3-D tiles load their aligned halo rows straight into LDS.
+    Those loads are counted by the vector-memory counter, not the LDS one, so
+    after them the wave waits for that counter to reach 0 before the
+    workgroup's next barrier; only then may other waves read the rows.  The
+    check is pinned by the same code with those waits taken out."""
+    _need_hipcc()
+    from xgrid_amd import workloads as W
+    from xgrid_amd.backend import lower_kernel
+    body = _kernel_body(_assembly(lower_kernel(W.make_heat_3d())[0].device_src), "xg_k0")
+    lines = [ln.strip() for ln in body.splitlines()
+             if ln.strip() and not ln.strip().startswith((";", ".")) and not ln.rstrip().endswith(":")]
+    assert sum(ln.startswith("global_load_lds_dwordx4") for ln in lines) >= 2
+    assert not _lds_loads_unwaited(lines)
+    planted = [ln for ln in lines if not (ln.startswith("s_waitcnt") and "vmcnt(0)" in ln)]
+    assert _lds_loads_unwaited(planted)
+
+
 def test_staged_lane_shifts_run_on_every_lane(double):
     """A lane shift reads another lane's register; run by a subset of lanes it
     reads 0 from the disabled ones.  In the generated source every shift is a

@@ -154,6 +154,17 @@ __device__ __forceinline__ void put(gptr<V> dst, const V& v) {
   else *dst = v;
 }
 
+/* One 16-byte global load per lane written straight into LDS, with no
+ * register in between (gfx950's dwordx4 LDS load): lane l's bytes land at
+ * dst + 16 l.  dst must be the same for every lane of the wave.  The data is
+ * in LDS once the wave's vector-memory counter reaches 0. */
+__device__ __forceinline__ void load_lds16(gptr<const void> src, void* dst) {
+  __builtin_amdgcn_global_load_lds(src, (__attribute__((address_space(3))) void*)dst, 16, 0, 0);
+}
+
+/* Wait for this wave's vector-memory loads, LDS loads above included. */
+__device__ __forceinline__ void wait_vm() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+
 /* Store one cell; a store outside the array is dropped. */
 template <class T, int D, int OV, bool NT = false>
 __device__ __forceinline__ void st(const xg_gview& g, int i0, int i1, int i2, const T& v) {

@@ -573,6 +573,7 @@ class _St3Plan:
             out.append(f"    for (int n = 0; n < {NP}; ++n) st.e{g}[n] = edge ? "
                        f"g{j}[xg::st3_plane(p0 + n + {o0}, sh) * P0 + row * P1 + ecol] : ({t})0;")
         off = 0
+        lds_dma = False
         self._offs = []
         for g, (j, o0) in enumerate(self.yg):
             t = self.types[j]
@@ -585,16 +586,19 @@ class _St3Plan:
                        f"st.q{j}[n + {o0 - self.pmin(j)}][k];")
             # the halo rows above and below the tile: 2 x NP rows of C columns over 256 work-items
             if K * self._size(j) == 16:
-                # K columns per work-item, one 16-byte load, where aligned as above
+                # K columns per work-item, one 16-byte load, where aligned as above,
+                # straight into LDS: a wave loads one halo row (64 lanes x 16 bytes =
+                # C columns), so its LDS row is the same for the whole wave
                 out.append(f"      if (full && ((uint64_t)R.v[{j}].ptr & 15) == 0 && (P1 % {K}) == 0) {{")
                 out.append("#pragma unroll")
                 out.append(f"        for (int h = 0; h < {2 * NP * 64 // 256}; ++h) {{")
-                out.append(f"          const int e = threadIdx.x + h * 256, cc = (e % 64) * {K}, "
+                out.append(f"          const int e = wr * 64 + h * 256, cc = lane * {K}, "
                            f"pl = (e / 64) % {NP}, hr = e / {64 * NP};")
                 out.append(f"          const int64_t gr = min(max(hr ? r0 + {self.ROWS} : r0 - 1, 0), sh[1] - 1);")
-                out.append(f"          *(xg::vecn<{t}, {K}>*)(L + (pl * {R2} + (hr ? {R2 - 1} : 0)) * {C} + cc) = "
-                           f"*(xg::gptr<const xg::vecn<{t}, {K}>>)(g{j} + (xg::st3_plane(p0 + pl + {o0}, sh) * P0 + gr * P1 + c0 + cc));")
+                out.append(f"          xg::load_lds16(g{j} + (xg::st3_plane(p0 + pl + {o0}, sh) * P0 + gr * P1 + c0 + cc), "
+                           f"L + (pl * {R2} + (hr ? {R2 - 1} : 0)) * {C});")
                 out.append("        }")
+                lds_dma = True
                 out.append("      } else {")
             else:
                 out.append("      {")
@@ -609,6 +613,8 @@ class _St3Plan:
             out.append("      }")
             out.append("    }")
             off += NP * R2 * C * self._size(j)
+        if lds_dma:
+            out.append("    xg::wait_vm();   /* the halo rows loaded straight into LDS */")
         if self.yg:
             out.append("    __syncthreads();")
         out.append("  }")
