@@ -120,17 +120,18 @@ def test_batched_diffusion_2d_equal_alone(double):
     batch.close()
 
 
-@pytest.mark.parametrize("prec", ["double", "float"])
-def test_batched_heat_3d_equal_alone(prec, request):
+@pytest.mark.parametrize("prec,max_blocks", [("double", 0), ("float", 0), ("double", 16), ("float", 16)])
+def test_batched_heat_3d_equal_alone(prec, max_blocks, request):
     """Batched 3-D statements (staged loads: 128-column tiles in FP64, 256 in
     FP32) on non-cubic grids, one narrower than a 64-column tile and one with two
     full tiles of either width, equal the same instances run alone and the numpy
-    restatement of the reference's C."""
+    restatement of the reference's C.  With a block cap the launch is persistent:
+    a workgroup stages tile after tile through the same LDS."""
     xg = request.getfixturevalue("double" if prec == "double" else "single")
     from xgrid_amd import workloads as W
     from xgrid_amd.batch import Batch
     kb, kp = W.make_heat_3d(), W.make_heat_3d()
-    batch = Batch(kb)
+    batch = Batch(kb, max_blocks=max_blocks)
     rng = np.random.default_rng(5)
     dt_np = np.float64 if prec == "double" else np.float32
     pairs = []

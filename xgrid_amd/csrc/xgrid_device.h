@@ -772,6 +772,10 @@ __device__ __forceinline__ int find_entry(const E* __restrict__ ents, int32_t n,
 template <class E, class F>
 __device__ __forceinline__ void for_batch_tiles(const E* __restrict__ ents, int32_t n, int64_t total, F&& f) {
   for (int64_t p = blockIdx.x; p < total; p += gridDim.x) {
+    /* A persistent workgroup's next tile rewrites LDS that its waves may still
+     * be reading for the last one (a staged 3-D tile's rows): wait for all of
+     * them first.  p is the same for every work-item of the workgroup. */
+    if (p != (int64_t)blockIdx.x) __syncthreads();
     const int64_t g = xcd_remap((uint32_t)p, (uint32_t)total);
     const E& e = ents[find_entry(ents, n, g)];
     f(e, (uint32_t)(g - e.tile0));
