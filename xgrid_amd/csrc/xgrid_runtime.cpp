@@ -20,6 +20,8 @@
 #include <mutex>
 #include <vector>
 
+#include <unistd.h>
+
 #include "xgrid_internal.h"
 #include "xgrid_kernel.h"
 
@@ -940,7 +942,10 @@ int xg_rtc_compile(const char* src, const char* name, const char* const* opts, i
   std::vector<char> code(code_n);
   hiprtcGetCode(prog, code.data());
   hiprtcDestroyProgram(&prog);
-  std::string tmp = std::string(out_path) + ".tmp" + std::to_string((long long)std::chrono::steady_clock::now().time_since_epoch().count());
+  // a name of this process and moment: processes sharing a cache may compile the
+  // same kernel at once, and each renames its own complete file into place
+  std::string tmp = std::string(out_path) + ".tmp" + std::to_string((long long)getpid()) + "." +
+                    std::to_string((long long)std::chrono::steady_clock::now().time_since_epoch().count());
   FILE* f = std::fopen(tmp.c_str(), "wb");
   if (!f) return fail(XG_ERR_IO, std::string("cannot write ") + tmp);
   const size_t w = std::fwrite(code.data(), 1, code.size(), f);
