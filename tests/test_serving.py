@@ -96,7 +96,10 @@ def test_budget_bounds_the_resident_set():
     results, _ = srv.run(trace)
     assert len(results) == 10
     assert srv.peak_used <= 2 * one
-    assert srv.peak_active <= 2
+    # the budget bounds bytes, not requests: how many fit depends on which
+    # requests have arrived together (wall-clock arrivals), so the bound on the
+    # count is the budget over the smallest footprint
+    assert 1 <= srv.peak_active <= (2 * one) // min(footprint(s, 4) for s in trace)
 
 
 def test_cell_cap_bounds_the_resident_work():

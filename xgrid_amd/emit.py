@@ -532,6 +532,12 @@ class _St3Plan:
         # partial last one) a lane's columns need no clamp; where their address is a
         # multiple of 16 bytes they are read as one 16-byte load.  (Marking the wave's
         # row uniform with readfirstlane made the FP64 heat kernel spill.)
+        # 8-byte tiles load at a raised wave priority, so a wave that starts its
+        # tile issues its loads ahead of other waves' arithmetic (512^3 FP64
+        # update 385.3-385.6 against 389.4-390.4 us; FP32 slower, 205-207 against 204)
+        prio = self.esize == 8
+        if prio:
+            out.append("    __builtin_amdgcn_s_setprio(1);")
         out.append("    const int lane = threadIdx.x & 63, wr = threadIdx.x >> 6;")
         out.append("    st.lds = lds; st.lane = lane; st.wr = wr;")
         out.append("    const int64_t P1 = sh[2], P0 = (int64_t)sh[1] * sh[2];")
@@ -617,6 +623,8 @@ class _St3Plan:
             out.append("    xg::wait_vm();   /* the halo rows loaded straight into LDS */")
         if self.yg:
             out.append("    __syncthreads();")
+        if prio:
+            out.append("    __builtin_amdgcn_s_setprio(0);")
         out.append("  }")
         # Every work-item of the workgroup runs this for every n, and the lane shifts come
         # before the edge lanes' selects: a shift written inside `lane == 63 ? e : shift`
